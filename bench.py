@@ -1,0 +1,175 @@
+"""Benchmark: decoded info Gbit/s and codewords/s of the int8 offset min-sum decoder,
+802.11ad rate 1/2 (N=672, K=336, Z=42), on one MI355X.
+
+Workload (the reference's default run, build/run_LDPCsim.pl:8-39 / README.md:41-47):
+beta 1, row cap 31, at most 10 iterations with syndrome early exit, channel LLRs
+quantized to int8 (scale 2, |LLR| <= 24) for SNR -1.00 .. 0.00 dB in 0.25 dB steps,
+1e6 codewords per SNR point.  One step = decoding all five SNR batches (5e6 cw),
+decode only, inputs resident in HBM (the reference times LDPCdecode_set alone,
+src/Top/LDPCtop.cpp:107-110).
+
+Prints one JSON line.  The CPU baseline (the oracle's SIMD restatement of
+LDPCtop_Vec64c_min on the host cores) is timed in the same run on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(REPO, "simd-fecfun_amd"), os.path.join(REPO, "oracle"), os.path.join(REPO, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "decoded info Gbit/s + CW/s, int8 min-sum 11ad 1/2; dB gain of learned offsets"
+SNRS = (-1.0, -0.75, -0.5, -0.25, 0.0)
+K_INFO = 336
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz, 32-bit lane-ops
+OPS_PER_EDGE_UPDATE = 13                    # algorithmic int ops per edge-update (offset min-sum)
+E_11AD = 2184                               # edges per iteration, 11ad 1/2
+
+
+def make_llrs(cs, ncw, snr, seed, torch, llr_scale=2.0, max_abs=24.0):
+    """Random codewords, BPSK over AWGN, quantized like gen_metrics.cpp:76-98, built on
+    the GPU with the library's encoder/noise path when present, else an all-zero
+    codeword with torch noise (equivalent error statistics for a linear code)."""
+    from refsim import O
+    sigma, f = O.snr_params(snr, llr_scale)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    n = torch.randn(ncw, cs.n, device="cuda", generator=g)
+    y = torch.addcmul(torch.ones_like(n), n, torch.tensor(sigma, device="cuda"))
+    llr = torch.clamp(y * f, -max_abs, max_abs)
+    return torch.round(llr).to(torch.int8).contiguous()
+
+
+def cpu_baseline(cs, llr_samples, max_seconds=12.0):
+    """Oracle SIMD decoder (LDPCtop_Vec64c_min restated) on the host cores."""
+    import numpy as np
+    from refsim import O
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    nthreads = max(1, min(ncores, 16))
+    bufs = [O.dense_to_ref(x, cs.z, cs.dec.mcol, 64) for x in llr_samples]
+    done_cw, t_dec, isa = 0, 0.0, 0
+    while t_dec < max_seconds:
+        for b in bufs:
+            work = b.copy()
+            t0 = time.perf_counter()
+            _, isa = O.decode_vec64c(cs.dec, work, nthreads=nthreads)
+            t_dec += time.perf_counter() - t0
+            done_cw += b.shape[0]
+    # one-thread figure on one SNR batch slice, for README.md:43-47
+    one = bufs[0][:20000].copy()
+    t0 = time.perf_counter()
+    O.decode_vec64c(cs.dec, one, nthreads=1)
+    t1 = time.perf_counter() - t0
+    return {
+        "value": done_cw / t_dec, "unit": "cw/s", "cores": nthreads, "kind": "port",
+        "sample": "%d cw decoded (%d-cw batches at SNR %s dB, repeated) by oracle/baseline_vec.c "
+                  "(AVX%d int8 x64 lanes), %.1f s of decode" % (done_cw, bufs[0].shape[0],
+                                                                 "/".join("%g" % s for s in SNRS), isa, t_dec),
+        "one_thread_cw_per_s": 20000 / t1,
+        "info_gbit_s": done_cw / t_dec * K_INFO / 1e9,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncw", type=int, default=1_000_000, help="codewords per SNR point")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(codeword batches shard trivially; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import numpy as np
+    from fecsim import codes
+    from fecsim.decoder import Decoder, Hdec
+
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+    ncw = args.ncw
+    print("bench: generating %d x %d codeword LLRs on the GPU" % (len(SNRS), ncw), file=sys.stderr, flush=True)
+    llrs = [make_llrs(cs, ncw, s, 100 + i, torch) for i, s in enumerate(SNRS)]
+    outs = [torch.empty_like(x) for x in llrs]
+    iters = [torch.zeros(ncw, dtype=torch.int32, device="cuda") for _ in SNRS]
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+
+    def step():
+        for x, o, it in zip(llrs, outs, iters):
+            dec.decode(x, o, iters=it, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps * len(SNRS))]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k = 0
+    for _ in range(args.steps):
+        for x, o, it in zip(llrs, outs, iters):
+            ev[k][0].record(stream)
+            dec.decode(x, o, iters=it, stream=stream)
+            ev[k][1].record(stream)
+            k += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+
+    tot_iter = int(sum(int(it.sum().item()) for it in iters))   # per step
+    cw_per_step = ncw * len(SNRS)
+    cw_s = cw_per_step * args.steps / wall
+    gbit = cw_s * K_INFO / 1e9
+    avg_launch_ms = float(np.mean(kern_ms))
+    # algorithmic HBM bytes per launch: N bytes LLR in + N bytes posterior out + 4 B iteration count per cw
+    hbm_bytes = ncw * (cs.n + cs.n + 4)
+    hbm_gbs = hbm_bytes / (avg_launch_ms * 1e-3) / 1e9
+    edge_updates_per_launch = tot_iter / len(SNRS) * E_11AD
+    valu_tops = edge_updates_per_launch * OPS_PER_EDGE_UPDATE / (avg_launch_ms * 1e-3) / 1e12
+    mean_iters = {"%g" % s: float(it.float().mean().item()) for s, it in zip(SNRS, iters)}
+
+    res = {
+        "metric": METRIC, "value": gbit, "unit": "info Gbit/s", "cw_per_s": cw_s,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": gbit / 30.2,
+        "vs_baseline_ref": "README.md:5 '90 million codewords/s' dual-socket Ice Lake = 30.2 info Gbit/s",
+        "dtype": "int8", "data": "synthetic",
+        "config": {"workload": "11ad 1/2 (N=672,K=336,Z=42) int8 offset min-sum decode, beta 1, cap 31, max 10 "
+                               "iterations with syndrome exit, SNR -1.00..0.00 dB step 0.25, %d cw per SNR" % ncw,
+                   "codewords_per_step": cw_per_step, "input": "all-zero codeword + AWGN, quantized LLRs"},
+        "mean_iterations": mean_iters,
+        "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_decode<IntPolicy<8>,8>", "avg_launch_ms": avg_launch_ms,
+                     "algorithmic_bytes_per_launch": hbm_bytes},
+        "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                          "frac": valu_tops / VALU_PEAK_TOPS,
+                          "ops_per_edge_update": OPS_PER_EDGE_UPDATE, "edge_updates_per_launch": edge_updates_per_launch},
+        "learned_offset_gain_db": None,
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        sample = [llrs[i][:100_000].cpu().numpy() for i in range(len(SNRS))]
+        cb = cpu_baseline(cs, sample, args.cpu_seconds)
+        res["cpu_baseline"] = cb
+        res["gpu_over_cpu"] = cw_s / cb["value"]
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

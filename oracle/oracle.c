@@ -431,7 +431,7 @@ static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t 
                             Q[j] = L[vidx[j]] - R[e * Z + r];                                                   \
                             FT a = FABS(Q[j]) - b;                                                              \
                             A[j] = a > (FT)0 ? a : (FT)0; /* _mm_max_ps(a, 0) */                                \
-                            sgn ^= (int)signbit(Q[j]);                                                          \
+                            sgn ^= (signbit(Q[j]) != 0);                                                        \
                         }                                                                                       \
                         if (mode == ORC_PROD) {                                                                 \
                             TREEPROD(w, A, cap, M);                                                             \

@@ -1,0 +1,66 @@
+"""ctypes binding of libfecsim.so (the C ABI of include/fecsim.h).
+
+The HIP library is the only implementation: if it is missing this module raises,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfecsim.so")
+
+I8, I16, I32, F32, F64 = 0, 1, 2, 3, 4
+MIN, PROD = 0, 1
+
+# (name, restype, argtypes) of every exported entry point
+_vp, _u32, _i32, _int = C.c_void_p, C.c_uint32, C.c_int32, C.c_int
+SIGNATURES = [
+    ("fecsim_abi_version", _int, []),
+    ("fecsim_strerror", C.c_char_p, [_int]),
+    ("fecsim_last_error", C.c_char_p, []),
+    ("fecsim_hdec_load", _int, [C.c_char_p, C.POINTER(_vp)]),
+    ("fecsim_hdec_from_words", _int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    ("fecsim_hdec_info", _int, [_vp, _vp]),
+    ("fecsim_hdec_free", None, [_vp]),
+    ("fecsim_decoder_create", _int, [_vp, C.c_char_p, _vp, C.POINTER(_vp)]),
+    ("fecsim_decoder_info", _int, [_vp, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
+    ("fecsim_decoder_free", None, [_vp]),
+    ("fecsim_decode_set", _int, [_vp, _u32, _int, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_decode", _int, [_vp, _u32, _int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_decode_set_host", _int, [_vp, _u32, _int, _vp, _vp, C.POINTER(C.c_ulonglong)]),
+]
+
+
+class FecsimError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = "%s failed: %s (%d)" % (where, _lib.fecsim_strerror(status).decode(), status)
+        detail = _lib.fecsim_last_error().decode()
+        if detail:
+            msg += ": " + detail
+        super().__init__(msg)
+
+
+_lib = None
+
+
+def lib():
+    """Load libfecsim.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libfecsim.so not built at %s (run simd-fecfun_amd/build.py)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, where):
+    if status != 0:
+        raise FecsimError(status, where)

@@ -103,29 +103,35 @@ def check_schedule(n: int, ops, out) -> None:
 
 
 def emit_hip_header(max_w: int = 32) -> str:
-    """Device functions `prod_tree_<w>(const float *t, float cap, float *o)`: o[j] =
-    product over k != j of t[k] in the reference's order, with the cap clamps."""
+    """Device functions `prod_tree_<w>(const F *t, F cap, F *o)` (F = float or double):
+    o[j] = product over k != j of t[k] in the reference's order, with the cap clamps
+    min(cap, m) = (cap < m ? cap : m), the operand order of vectorclass min()."""
     lines = ["// Generated by fecsim/minprod_tree.py (restating src/LDPCDec/min_find.pl). Do not edit.",
-             "#pragma once", ""]
+             "#pragma once", "",
+             "template <typename F> __device__ __forceinline__ F vmin_ref(F a, F b) { return a < b ? a : b; }", ""]
     for n in range(2, max_w + 1):
         ops, out = tree_schedule(n)
-        lines.append("__device__ __forceinline__ void prod_tree_%d(const float* t, float cap, float* o) {" % n)
+        lines.append("template <typename F> __device__ __forceinline__ void prod_tree_%d(const F* t, F cap, F* o) {" % n)
         for i in range(n):
-            lines.append("  float m%x = t[%d];" % (1 << i, i))
+            lines.append("  F m%x = t[%d];" % (1 << i, i))
         seen = {1 << i for i in range(n)}
         for d, a, b in ops:
             if a == CAP:
-                lines.append("  m%x = fminf_ref(cap, m%x);" % (d, d))
+                lines.append("  m%x = vmin_ref(cap, m%x);" % (d, d))
             else:
-                decl = "" if d in seen else "float "
+                decl = "" if d in seen else "F "
                 seen.add(d)
                 lines.append("  %sm%x = m%x * m%x;" % (decl, d, a, b))
         for j, m in enumerate(out):
             lines.append("  o[%d] = m%x;" % (j, m))
         lines.append("}")
         lines.append("")
-    lines.append("template <int W> __device__ __forceinline__ void prod_tree(const float* t, float cap, float* o);")
+    lines.append("// run-time row weight -> straight-line schedule")
+    lines.append("template <typename F> __device__ __forceinline__ void prod_tree_dyn(int w, const F* t, F cap, F* o) {")
+    lines.append("  switch (w) {")
     for n in range(2, max_w + 1):
-        lines.append("template <> __device__ __forceinline__ void prod_tree<%d>(const float* t, float cap, float* o) "
-                     "{ prod_tree_%d(t, cap, o); }" % (n, n))
+        lines.append("  case %d: prod_tree_%d<F>(t, cap, o); break;" % (n, n))
+    lines.append("  default: break;")
+    lines.append("  }")
+    lines.append("}")
     return "\n".join(lines) + "\n"

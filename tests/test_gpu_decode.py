@@ -1,0 +1,172 @@
+"""GPU decoder parity: the HIP layered decoders (through the C ABI) against the CPU
+oracle -- bit-exact posteriors, C2V messages and iteration counts."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from fecsim import codes
+from fecsim.decoder import Decoder, Hdec
+import refsim
+from refsim import O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _bytes_eq(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+def _gpu_decode(dec_tables, kind, llr, max_iter, mrm, beta, layout_p=0, want_c2v=True, inplace=False):
+    import torch
+    h = Hdec.from_tables(dec_tables)
+    d = Decoder(h, kind, max_iter, mrm, beta)
+    x = torch.from_numpy(np.ascontiguousarray(llr)).cuda()
+    ncw = llr.shape[0]
+    iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    c2v = torch.zeros(ncw * h.e, dtype=d.torch_dtype, device="cuda") if want_c2v else None
+    if inplace:
+        d.decode_set(x, iters=iters, c2v=c2v, tot_iter=tot, layout_p=layout_p)
+        out = x
+    else:
+        out = torch.empty_like(x)
+        d.decode(x, out, iters=iters, c2v=c2v, tot_iter=tot, layout_p=layout_p)
+        assert torch.equal(x.cpu(), torch.from_numpy(np.ascontiguousarray(llr)))  # input untouched
+    torch.cuda.synchronize()
+    c = None if c2v is None else c2v.cpu().numpy().reshape(ncw, -1)
+    return out.cpu().numpy(), c, iters.cpu().numpy().astype(np.uint32), int(tot.item())
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "dec_*.npz"))), ids=os.path.basename)
+def test_gpu_matches_golden(path, cuda):
+    f = np.load(path)
+    _say("golden", os.path.basename(path), str(f["kind"]), f["llr"].shape)
+    dec = codes.DecTables.from_bytes(f["dat"].tobytes())
+    mi, mrm, beta = f["params"]
+    post, c2v, iters, tot = _gpu_decode(dec, str(f["kind"]), f["llr"], int(mi), float(mrm), float(beta))
+    assert np.array_equal(iters, f["iters"])
+    assert _bytes_eq(post, f["post"])
+    assert _bytes_eq(c2v, f["c2v"])
+    assert tot == int(f["iters"].sum())
+
+
+LIVE = [
+    # MatrixSet, num_par, kind, snr, ncw, z_over
+    ("11ad 1/2", 1, "Vec64c_min", -1.0, 600, None),
+    ("11ad 1/2", 1, "Vec64c_min", 0.0, 600, None),
+    ("11ad 1/2", 6, "Vec64c_min", -0.5, 100, None),
+    ("11n 1/2 z27", 1, "Vec64c_min", -0.5, 300, None),
+    ("11n 3/4 z54", 1, "Vec64c_min", 2.0, 200, None),
+    ("11n 5/6 z81", 1, "Vec64c_min", 3.5, 200, None),
+    ("11ad 13/16", 1, "Vec64c_min", 3.0, 300, None),
+    ("16e 2/3A", 1, "Vec64c_min", 1.0, 100, 96),
+    ("11n 1/2 z81", 1, "Vec64c_min", -0.5, 200, None),
+    ("16e 1/2", 1, "Vec16f_prod", -1.0, 60, 48),
+    ("16e 1/2", 1, "Vec16f_prod", -0.5, 60, 96),
+    ("11n 1/2 z81", 1, "Vec16f_min", -1.0, 60, None),
+    ("11ad 1/2", 1, "Vec32s_min", -0.5, 100, None),
+]
+
+
+@pytest.mark.parametrize("case", LIVE, ids=lambda c: "%s-np%d-%s-%s" % c[:4])
+def test_gpu_matches_oracle_reference_chain(case, cuda):
+    """LLRs from the reference chain (sfc64 WGN, xoshiro bits, encoder, gen_metrics)."""
+    ms, npar, kind, snr, ncw, zo = case
+    _say("live", case)
+    p, ch, mode, ss, es = codes.parse_exec_name(kind)
+    cs = codes.gen_structs(ms, num_par=npar, simd_size=ss, elem_size=es, z_over=zo)
+    elem = O.ELEM_OF_CHAR[ch]
+    fl = elem >= O.F32
+    mrm, beta, scale, mx = (6e-8, 0.0, 1.0, 1e9) if mode == "prod" else ((31.0, 0.5, 1.0, 1e9) if fl else (31.0, 1.0, 2.0, 24.0))
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, snr, elem=elem, llr_scale=scale, max_abs=mx)
+    ref = O.decode_ideal(cs.dec, llr, elem=elem, mode=O.PROD if mode == "prod" else O.MIN, max_row_metric=mrm,
+                         beta=beta)
+    got = _gpu_decode(cs.dec, kind, llr, 10, mrm, beta)
+    assert np.array_equal(got[2], ref[2])
+    assert _bytes_eq(got[0], ref[0])
+    assert _bytes_eq(got[1], ref[1])
+    assert got[3] == ref[3]
+
+
+def test_gpu_random_int8_saturation_corners(cuda):
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    rng = np.random.default_rng(2)
+    llr = rng.integers(-128, 128, size=(257, cs.n)).astype(np.int8)
+    llr[::3, ::5] = -128
+    for beta, cap in ((1.0, 31.0), (0.0, 127.0), (3.0, 9.0)):
+        ref = O.decode_ideal(cs.dec, llr, max_iter=7, max_row_metric=cap, beta=beta)
+        got = _gpu_decode(cs.dec, "Vec64c_min", llr, 7, cap, beta)
+        assert np.array_equal(got[2], ref[2]) and np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_gpu_reference_layout_and_inplace(cuda):
+    """layout_p = 64 decodes the reference's padded buffer in place; guard words are
+    not touched and the data equals the dense decode."""
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(150)
+    dense, _ = rs.metrics(wgn, bits, -0.75, P=0)
+    padded, _ = rs.metrics(wgn, bits, -0.75, P=64)
+    garbage = np.random.default_rng(9).integers(-128, 128, size=padded.shape).astype(np.int8)
+    spz = 1
+    mask = np.zeros((150, cs.dec.mcol, (spz + 2) * 64), bool)
+    mask[:, :, 64:64 + cs.z] = True
+    padded = np.where(mask.reshape(150, -1), padded, garbage)
+    ref = O.decode_ideal(cs.dec, dense)
+    got_p = _gpu_decode(cs.dec, "Vec64c_min", padded, 10, 31.0, 1.0, layout_p=64, inplace=True, want_c2v=False)
+    got_d = _gpu_decode(cs.dec, "Vec64c_min", dense, 10, 31.0, 1.0, inplace=True)
+    assert np.array_equal(O.ref_to_dense(got_p[0], cs.z, cs.dec.mcol, 64), ref[0])
+    assert np.array_equal(got_p[0].reshape(150, -1)[~mask.reshape(150, -1)], garbage[~mask.reshape(150, -1)])
+    assert np.array_equal(got_d[0], ref[0]) and np.array_equal(got_d[2], ref[2])
+
+
+def test_gpu_decode_set_host(cuda):
+    cs = codes.gen_structs("11n 1/2 z27", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(64)
+    llr, _ = rs.metrics(wgn, bits, -0.5, P=64)
+    want = O.decode_ideal(cs.dec, O.ref_to_dense(llr, cs.z, cs.dec.mcol, 64))
+    d = Decoder(Hdec.from_tables(cs.dec), "LDPCtop_Vec64c_min", 10, 31.0, 1.0)
+    m = llr.copy()
+    tot, iters = d.decode_set_host(m, layout_p=64)
+    assert tot == want[3] and np.array_equal(iters, want[2])
+    assert np.array_equal(O.ref_to_dense(m, cs.z, cs.dec.mcol, 64), want[0])
+    assert d.decode_set_host(m[:0], layout_p=64)[0] == 0
+
+
+def test_gpu_large_batch_properties(cuda):
+    """Full-size batch (11ad 1/2, 200k codewords at -1 dB): every codeword that
+    stopped early has a zero syndrome; a sample equals the oracle."""
+    import torch
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    ncw = 200_000
+    s, f = O.snr_params(-1.0, 2.0)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    y = 1.0 + s * torch.randn(ncw, cs.n, device="cuda", generator=g)
+    llr = torch.clamp(y * f, -24, 24).round().to(torch.int8)
+    h = Hdec.from_tables(cs.dec)
+    d = Decoder(h, "Vec64c_min", 10, 31.0, 1.0)
+    out = torch.empty_like(llr)
+    iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    d.decode(llr, out, iters=iters)
+    torch.cuda.synchronize()
+    hd = (out < 0).to(torch.int32).cpu().numpy()
+    H = codes.expand_h(cs.hc, cs.z).astype(np.int32)
+    it = iters.cpu().numpy()
+    early = it < 10
+    syn = (hd[early][:5000] @ H.T) & 1
+    assert not syn.any()
+    idx = np.random.default_rng(0).choice(ncw, 3000, replace=False)
+    ref = O.decode_ideal(cs.dec, llr.cpu().numpy()[idx])
+    assert np.array_equal(out.cpu().numpy()[idx], ref[0]) and np.array_equal(it[idx].astype(np.uint32), ref[2])
+    _say("mean iterations at -1 dB:", it.mean())
