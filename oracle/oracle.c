@@ -1048,3 +1048,29 @@ uint32_t orc_noise_llr(uint64_t seed, uint64_t cw0, uint32_t num_cw, const uint8
     }
     return err;
 }
+
+/* the product's counter-based data bits: 64-bit chunk k of data column c of codeword
+ * cw is splitmix64((seed ^ DATA_KEY) + (q + 1) * gamma), q = (cw*mcol + c)*8 + k,
+ * masked to Z bits and to non-filler bits (the role of gen_rand_databits.cpp:11-40) */
+#define DATA_KEY 0xd1b54a32d192ed03ull
+void orc_gpu_databits(uint64_t seed, uint64_t cw0, uint32_t num_cw, int z, int mrow, int mcol,
+                      const uint8_t *filler, uint8_t *bits) {
+    for (uint32_t cw = 0; cw < num_cw; ++cw) {
+        uint8_t *base = bits + (size_t)cw * mcol * WB;
+        memset(base, 0, (size_t)mcol * WB);
+        for (int c = 0; c < mcol - mrow; ++c) {
+            uint64_t w[8];
+            int zz = z;
+            for (int k = 0; k < 8; ++k) {
+                uint64_t q = (((uint64_t)(cw0 + cw) * (uint64_t)mcol + (uint64_t)c) * 8u + (uint64_t)k);
+                uint64_t v = orc_splitmix64((seed ^ DATA_KEY) + (q + 1) * GOLDEN_GAMMA);
+                uint64_t m = zz >= 64 ? ~0ull : (zz <= 0 ? 0 : (~0ull >> (64 - zz)));
+                zz = zz >= 64 ? zz - 64 : 0;
+                uint64_t fm;
+                memcpy(&fm, filler + (size_t)c * WB + 8 * k, 8);
+                w[k] = v & m & ~fm;
+            }
+            memcpy(base + (size_t)c * WB, w, WB);
+        }
+    }
+}

@@ -107,6 +107,10 @@ uint32_t orc_noise_llr(uint64_t seed, uint64_t cw0, uint32_t num_cw, const uint8
                        float sigma, float factor, float max_abs_llr, const uint8_t *punct, const uint8_t *fill,
                        int elem, void *llr);
 
+/* the product's counter-based data bits (words layout as orc_gen_rand_databits) */
+void orc_gpu_databits(uint64_t seed, uint64_t cw0, uint32_t num_cw, int z, int mrow, int mcol,
+                      const uint8_t *filler_mask, uint8_t *cw_bits);
+
 /* own float/double math used by the sum-product decoder (parity unpinned against
  * vectorclass's tanh/atanh; see the header comment of oracle.c) */
 float orc_tanhf(float x);

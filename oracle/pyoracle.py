@@ -70,6 +70,7 @@ def lib():
         L.orc_noise_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_noise_llr.restype = u32
         L.orc_noise_llr.argtypes = [C.c_uint64, C.c_uint64, u32, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp, vp, C.c_int, vp]
+        L.orc_gpu_databits.argtypes = [C.c_uint64, C.c_uint64, u32, C.c_int, C.c_int, C.c_int, vp, vp]
         for f in ("orc_tanhf", "orc_atanhf"):
             getattr(L, f).restype = C.c_float
             getattr(L, f).argtypes = [C.c_float]

@@ -1,0 +1,259 @@
+// Device core of the layered LDPC decoders (shared by the plain decoder kernel and
+// the fused simulation kernel).
+//
+// Semantics restated from src/LDPCDec/LDPCdecode.cpp:79-201 and
+// decode_functions.cpp:11-137 (per check row r of layer l, edge j on column c_j
+// with relabeled shift s_j, variable v_j = c_j*Z + (s_j + r) mod Z):
+//   Q_j = sat(L[v_j] - R_j);  A_j = max(wrap(wabs(Q_j) - beta), 0);  S = xor of signs
+//   M_j = min(cap, min_{k!=j} A_k)          (min-sum, exact whatever the order)
+//   M_j = 2 atanh(prod-tree_{k!=j} tanh(A_k/2), cap clamps)   (sum-product)
+//   R_j = (S xor sign Q_j) ? wrap(-M_j) : M_j;  L[v_j] = sat(Q_j + R_j)
+// then after every full iteration the syndrome (sign bits) decides early exit.
+//
+// MI355X mapping: one wave (64 lanes) owns `cw_per_wave` codewords; a codeword's
+// posterior and C2V messages live in that wave's LDS slice, its Z check rows are
+// spread over `lanes_per_cw` lanes (rows_per_lane each).  A codeword never spans two
+// waves, so layers need no workgroup barrier: LDS operations of one wave complete in
+// order, and a wavefront-scope fence keeps the compiler from reordering them.
+// Early exit is a 64-bit ballot per iteration; lanes of finished codewords idle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <limits>
+#include <type_traits>
+
+#include "fecmath.h"
+#include "internal.h"
+#include "prodtree_gen.h"
+
+namespace fecsim {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// element policies
+// ---------------------------------------------------------------------------
+
+template <int BITS>
+struct IntPolicy {
+  using S = typename std::conditional<BITS == 8, int8_t,
+                                      typename std::conditional<BITS == 16, int16_t, int32_t>::type>::type;
+  using C = typename std::conditional<BITS == 32, long long, int>::type;   // compute type
+  using K = typename std::conditional<BITS == 32, long long, int>::type;   // min key type
+  static constexpr bool kFloat = false;
+  static constexpr bool kProd = false;
+  static constexpr C LO = -(C(1) << (BITS - 1));
+  static constexpr C HI = (C(1) << (BITS - 1)) - 1;
+  __device__ static __forceinline__ C sat(C x) { return x < LO ? LO : (x > HI ? HI : x); }
+  __device__ static __forceinline__ C wrap(C x) { return (C)(S)x; }
+  __device__ static __forceinline__ C wabs(C x) { return x == LO ? LO : (x < 0 ? -x : x); }
+};
+
+template <typename F, bool PROD>
+struct FloatPolicy {
+  using S = F;
+  static constexpr bool kFloat = true;
+  static constexpr bool kProd = PROD;
+};
+
+__device__ __forceinline__ bool sign_of(float x) { return (__float_as_uint(x) >> 31) != 0; }
+__device__ __forceinline__ bool sign_of(double x) { return __double_as_longlong(x) < 0; }
+
+template <class P>
+__device__ __forceinline__ bool is_neg(typename P::S x) {
+  if constexpr (P::kFloat) return sign_of(x);
+  else return x < 0;
+}
+
+// ---------------------------------------------------------------------------
+// one check row, integer elements (min-sum)
+// ---------------------------------------------------------------------------
+
+template <class P, int MAXW>
+__device__ __forceinline__ void row_int(typename P::S* __restrict__ post, typename P::S* __restrict__ c2v,
+                                        const uint32_t* __restrict__ edges, int e0, int w, int r, int Z,
+                                        typename P::C beta, typename P::C cap) {
+  using C = typename P::C;
+  using K = typename P::K;
+  C q[MAXW];
+  int vi[MAXW];
+  K m1 = std::numeric_limits<K>::max(), m2 = std::numeric_limits<K>::max();
+  int sg = 0;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const uint32_t ed = edges[e0 + j];
+      int v = (int)(ed & 0xffffu) + r;
+      v = v >= Z ? v - Z : v;
+      v += (int)(ed >> 16);
+      vi[j] = v;
+      const C qq = P::sat((C)post[v] - (C)c2v[(e0 + j) * Z + r]);
+      q[j] = qq;
+      C a = P::wrap(P::wabs(qq) - beta);
+      a = a > 0 ? a : 0;
+      sg ^= (qq < 0);
+      const K key = ((K)a << 6) | j;
+      const K hi = key > m1 ? key : m1;
+      m2 = hi < m2 ? hi : m2;
+      m1 = key < m1 ? key : m1;
+    }
+  }
+  const int im = (int)(m1 & 63);
+  C M1 = (C)(m1 >> 6), M2 = (C)(m2 >> 6);
+  M1 = cap < M1 ? cap : M1;
+  M2 = cap < M2 ? cap : M2;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const C m = (j == im) ? M2 : M1;
+      const bool neg = sg ^ (q[j] < 0);
+      const C rn = neg ? P::wrap(-m) : m;
+      post[vi[j]] = (typename P::S)P::sat(q[j] + rn);
+      c2v[(e0 + j) * Z + r] = (typename P::S)rn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one check row, float elements (min-sum or sum-product)
+// ---------------------------------------------------------------------------
+
+template <class P, int MAXW>
+__device__ __forceinline__ void row_float(typename P::S* __restrict__ post, typename P::S* __restrict__ c2v,
+                                          const uint32_t* __restrict__ edges, int e0, int w, int r, int Z,
+                                          typename P::S beta, typename P::S cap) {
+  using F = typename P::S;
+  F q[MAXW], A[MAXW], M[MAXW];
+  int vi[MAXW];
+  int sg = 0;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const uint32_t ed = edges[e0 + j];
+      int v = (int)(ed & 0xffffu) + r;
+      v = v >= Z ? v - Z : v;
+      v += (int)(ed >> 16);
+      vi[j] = v;
+      const F qq = post[v] - c2v[(e0 + j) * Z + r];
+      q[j] = qq;
+      const F a = fabs(qq) - beta;
+      A[j] = a > F(0) ? a : F(0);   // _mm_max_ps(a, 0)
+      sg ^= (int)sign_of(qq);
+    }
+  }
+  if constexpr (P::kProd) {
+    F t[MAXW];
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) t[j] = (j < w) ? fecmath::tanh_own(A[j] * F(0.5)) : F(0);
+    F o[MAXW];
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) o[j] = F(0);
+    prod_tree_dyn<F>(w, t, cap, o);
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) M[j] = (j < w) ? F(2) * fecmath::atanh_own(o[j]) : F(0);
+  } else {
+    F m1 = F(INFINITY), m2 = F(INFINITY);
+    int im = -1;
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) {
+      if (j < w) {
+        const F a = A[j];
+        const F hi = a > m1 ? a : m1;
+        m2 = hi < m2 ? hi : m2;
+        im = a < m1 ? j : im;
+        m1 = a < m1 ? a : m1;
+      }
+    }
+    const F M1 = cap < m1 ? cap : m1, M2 = cap < m2 ? cap : m2;
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) M[j] = (j == im) ? M2 : M1;
+  }
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const bool neg = sg ^ (int)sign_of(q[j]);
+      const F rn = neg ? -M[j] : M[j];
+      post[vi[j]] = q[j] + rn;
+      c2v[(e0 + j) * Z + r] = rn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the per-wave decode loop: every lane returns its codeword's iteration count
+// ---------------------------------------------------------------------------
+
+template <class P, int MAXW>
+__device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S* post, typename P::S* c2v,
+                                                int slot, int lr, bool valid) {
+  using S = typename P::S;
+  const int L = a.lanes_per_cw, Z = a.z;
+  if (valid)
+    for (int i = lr; i < a.e; i += L) c2v[i] = S(0);   // LDPCdecode.cpp:115-119
+  wave_sync();
+  const S fbeta = P::kFloat ? (S)a.fbeta : S(0), fcap = P::kFloat ? (S)a.fcap : S(0);
+  uint32_t iter = 0;
+  bool active = valid;
+  const unsigned long long gmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
+  while (__ballot(active) != 0ull) {
+    bool fail = false;
+    if (active) {
+      int e0 = 0;
+      for (int l = 0; l < a.mrow; ++l) {
+        const int w = a.layer_w[l];
+        for (int k = 0; k < a.rows_per_lane; ++k) {
+          const int r = lr + k * L;
+          if (r < Z) {
+            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, fbeta, fcap);
+            else row_int<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, (typename P::C)a.ibeta, (typename P::C)a.icap);
+          }
+        }
+        e0 += w;
+        wave_sync();
+      }
+      e0 = 0;
+      for (int l = 0; l < a.mrow; ++l) {                  // LDPCdecode.cpp:172-195
+        const int w = a.layer_w[l];
+        for (int k = 0; k < a.rows_per_lane; ++k) {
+          const int r = lr + k * L;
+          if (r < Z) {
+            bool par = false;
+            for (int j = 0; j < w; ++j) {
+              const uint32_t ed = a.edges[e0 + j];
+              int v = (int)(ed & 0xffffu) + r;
+              v = v >= Z ? v - Z : v;
+              v += (int)(ed >> 16);
+              par ^= is_neg<P>(post[v]);
+            }
+            fail |= par;
+          }
+        }
+        e0 += w;
+      }
+    }
+    const unsigned long long fb = __ballot(active && fail);
+    if (active) {
+      ++iter;
+      const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;
+      if (!cwfail || iter == a.max_iter) active = false;   // LDPCdecode.cpp:192-197
+    }
+  }
+  wave_sync();
+  return iter;
+}
+
+// dispatch on the row-weight bucket of a decoder
+#define FECSIM_MAXW_DISPATCH(maxw, KERNEL, P, grid, block, lds, st, ...)                 \
+  switch (maxw) {                                                                         \
+  case 8: hipLaunchKernelGGL((KERNEL<P, 8>), grid, block, lds, st, __VA_ARGS__); break;   \
+  case 16: hipLaunchKernelGGL((KERNEL<P, 16>), grid, block, lds, st, __VA_ARGS__); break; \
+  case 24: hipLaunchKernelGGL((KERNEL<P, 24>), grid, block, lds, st, __VA_ARGS__); break; \
+  default: hipLaunchKernelGGL((KERNEL<P, 32>), grid, block, lds, st, __VA_ARGS__); break; \
+  }
+
+}  // namespace fecsim
