@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--ncw", type=int, default=1_000_000, help="codewords per SNR point")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--kernel", choices=["auto", "generic", "pair"], default="auto",
+                    help="decoder kernel (results are identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -100,6 +102,9 @@ def main():
 
     cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
     dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+    from fecsim import lib as _L
+    dec.select_kernel({"auto": _L.KERNEL_AUTO, "generic": _L.KERNEL_GENERIC, "pair": _L.KERNEL_PAIR}[args.kernel])
+    kname = "k_pair_i8<PairShape0> (row-pair, persistent)" if dec.kernel == _L.KERNEL_PAIR else "k_decode<IntPolicy<8>,8>"
     ncw = args.ncw
     print("bench: generating %d x %d codeword LLRs on the GPU" % (len(SNRS), ncw), file=sys.stderr, flush=True)
     llrs = [make_llrs(cs, ncw, s, 100 + i, torch) for i, s in enumerate(SNRS)]
@@ -155,7 +160,7 @@ def main():
         "mean_iterations": mean_iters,
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_decode<IntPolicy<8>,8>", "avg_launch_ms": avg_launch_ms,
+                     "kernel": kname, "avg_launch_ms": avg_launch_ms,
                      "algorithmic_bytes_per_launch": hbm_bytes},
         "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                           "frac": valu_tops / VALU_PEAK_TOPS,

@@ -84,6 +84,15 @@ typedef struct {
  * Makefile:11-14 + simd_defs.h).  The tables are copied to the device. */
 int fecsim_decoder_create(const fecsim_hdec *h, const char *kind, const fecsim_dec_params *params,
                           fecsim_decoder **out);
+/* Kernel selection.  AUTO picks the row-pair int8 kernel when the code shape was
+ * compiled for it (even Z <= 64, int8 min-sum, 1 <= beta, 0 <= cap, dense layout) and
+ * the generic kernel otherwise; both give identical results.  No reference
+ * counterpart (the reference selects its decoder at compile time only). */
+enum { FECSIM_KERNEL_AUTO = 0, FECSIM_KERNEL_GENERIC = 1, FECSIM_KERNEL_PAIR = 2 };
+int fecsim_decoder_select_kernel(fecsim_decoder *d, int kernel);
+/* kernel a dense decode_set call would run now (FECSIM_KERNEL_GENERIC or _PAIR) */
+int fecsim_decoder_kernel(const fecsim_decoder *d);
+
 /* element kind (FECSIM_I8..F64), mode (FECSIM_MIN/PROD), bytes per element */
 int fecsim_decoder_info(const fecsim_decoder *d, int *elem, int *mode, int *elem_bytes);
 void fecsim_decoder_free(fecsim_decoder *d);

@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libfecsim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["decode.hip", "api.cpp"]
+SOURCES = ["decode.hip", "pair_i8.hip", "api.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -39,8 +39,17 @@ def _compile(src: str) -> str:
     return obj
 
 
+def generate() -> None:
+    """Code generation steps (shape list of the row-pair kernel, product-tree schedules)."""
+    sys.path.insert(0, HERE)
+    from fecsim import gen_pair_shapes, minprod_tree
+    gen_pair_shapes.emit(CSRC)
+    gen_pair_shapes._write(os.path.join(CSRC, "prodtree_gen.h"), minprod_tree.emit_hip_header(32))
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
+    generate()
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"

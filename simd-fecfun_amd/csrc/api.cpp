@@ -114,13 +114,6 @@ int parse_kind(const char* kind, int* elem, int* mode) {
 
 const int kElemBytes[5] = {1, 2, 4, 4, 8};
 
-long long wrap_to(long long x, int bits) {
-  const unsigned long long m = (1ull << bits) - 1;
-  unsigned long long u = (unsigned long long)x & m;
-  if (u >> (bits - 1)) u |= ~m;
-  return (long long)u;
-}
-
 }  // namespace
 
 extern "C" {
@@ -198,7 +191,7 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
   d.elem_bytes = kElemBytes[elem];
   d.params = *prm;
   std::string why;
-  rc = choose_shape(d.h, d.elem_bytes, &d.shape, &why);
+  rc = choose_shape(d.h, d.elem_bytes, 0, &d.shape, &why);
   if (rc) {
     delete dec;
     return fail(rc, "decoder shape: " + why);
@@ -217,8 +210,35 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
     fecsim_decoder_free(dec);
     return hip_fail(he, "decoder tables to device");
   }
+  d.pair_shape = pair_i8_shape(d);
+  if (d.pair_shape >= 0) {
+    pair_layout(h, &d.pair);
+    he = hipMalloc(&d.d_pair_adr, sizeof(uint32_t) * d.pair.adrtab.size());
+    if (he == hipSuccess)
+      he = hipMemcpy(d.d_pair_adr, d.pair.adrtab.data(), sizeof(uint32_t) * d.pair.adrtab.size(),
+                     hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+      fecsim_decoder_free(dec);
+      return hip_fail(he, "pair kernel tables to device");
+    }
+  }
   *out = dec;
   return FECSIM_OK;
+}
+
+int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
+  if (!d) return fail(FECSIM_E_INVAL, "null argument");
+  if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_PAIR) return fail(FECSIM_E_INVAL, "unknown kernel");
+  if (kernel == FECSIM_KERNEL_PAIR && d->d.pair_shape < 0)
+    return fail(FECSIM_E_UNSUPPORTED, "no row-pair kernel for this code / element / parameters");
+  d->d.kernel = kernel;
+  return FECSIM_OK;
+}
+
+int fecsim_decoder_kernel(const fecsim_decoder* d) {
+  if (!d) return fail(FECSIM_E_INVAL, "null argument");
+  if (d->d.kernel == FECSIM_KERNEL_GENERIC || d->d.pair_shape < 0) return FECSIM_KERNEL_GENERIC;
+  return FECSIM_KERNEL_PAIR;
 }
 
 int fecsim_decoder_info(const fecsim_decoder* d, int* elem, int* mode, int* elem_bytes) {
@@ -233,6 +253,7 @@ void fecsim_decoder_free(fecsim_decoder* d) {
   if (!d) return;
   if (d->d.d_edges) (void)hipFree(d->d.d_edges);
   if (d->d.d_layer_w) (void)hipFree(d->d.d_layer_w);
+  if (d->d.d_pair_adr) (void)hipFree(d->d.d_pair_adr);
   delete d;
 }
 
@@ -244,44 +265,12 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   const DecoderDev& d = dec->d;
   const HdecHost& h = d.h;
   KernelArgs a{};
-  a.edges = d.d_edges;
-  a.layer_w = d.d_layer_w;
-  a.mrow = h.mrow;
-  a.mcol = h.mcol;
-  a.z = h.z;
-  a.n = h.mcol * h.z;
-  a.e = h.total_hc_ones * h.z;
-  a.lanes_per_cw = d.shape.lanes_per_cw;
-  a.cw_per_wave = d.shape.cw_per_wave;
-  a.waves = d.shape.waves;
-  a.rows_per_lane = d.shape.rows_per_lane;
-  a.post_stride = d.shape.post_stride;
-  a.c2v_stride = d.shape.c2v_stride;
-  a.wave_bytes = d.shape.wave_bytes;
+  fill_core(d, &a.core);
   a.num_cw = num_cw;
-  a.max_iter = d.params.max_iter;
-  const float mrm = d.params.max_row_metric, beta = d.params.beta_offset;
-  if (d.elem <= FECSIM_I32) {
-    // decode_functions.cpp:28-32,60: float -> int truncation, then element broadcast
-    const int bits = 8 * d.elem_bytes;
-    a.ibeta = wrap_to((long long)(int)beta, bits);
-    if (mrm < 1e-6f) {
-      const long long t = wrap_to((long long)(int)mrm, bits);
-      a.icap = wrap_to((long long)(int)(1.0 - (double)t), bits);
-    } else {
-      a.icap = wrap_to((long long)(int)mrm, bits);
-    }
-  } else if (d.elem == FECSIM_F32) {
-    a.fbeta = (double)beta;
-    a.fcap = (double)(mrm < 1e-6f ? (float)(1.0 - (double)mrm) : mrm);
-  } else {
-    a.fbeta = (double)beta;
-    a.fcap = mrm < 1e-6f ? 1.0 - (double)mrm : (double)mrm;
-  }
   a.metrics_in = llr_in;
   a.metrics_out = post_out;
   if (layout_p == 0) {
-    a.cw_stride = a.n;
+    a.cw_stride = a.core.n;
     a.col_stride = h.z;
     a.base = 0;
   } else {
@@ -293,7 +282,8 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   a.iters = iters;
   a.c2v = c2v;
   a.tot_iter = tot_iter;
-  hipError_t he = launch_decode(d, a, (hipStream_t)stream);
+  const bool pair = layout_p == 0 && d.pair_shape >= 0 && d.kernel != FECSIM_KERNEL_GENERIC;
+  hipError_t he = pair ? launch_pair_i8(d, a, (hipStream_t)stream) : launch_decode(d, a, (hipStream_t)stream);
   if (he != hipSuccess) return hip_fail(he, "decode launch");
   return FECSIM_OK;
 }

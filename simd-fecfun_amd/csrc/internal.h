@@ -86,6 +86,12 @@ struct SimArgs {
   unsigned long long* counters;   // [nsnr][FECSIM_NCOUNT]
 };
 
+// Layout of the row-pair int8 kernel (pair_i8.hip)
+struct PairLayout {
+  int half = 0, cw_per_wave = 0, col_dwords = 0, slot_dwords = 0, waves = 0;
+  std::vector<uint32_t> adrtab;   // [half][total_hc_ones]
+};
+
 struct DecoderDev {
   HdecHost h;
   int elem = FECSIM_I8, mode = FECSIM_MIN, elem_bytes = 1;
@@ -93,8 +99,15 @@ struct DecoderDev {
   DecShape shape;
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
+  int pair_shape = -1;            // compiled code shape for the pair kernel, -1 if none
+  PairLayout pair;
+  uint32_t* d_pair_adr = nullptr;
+  int kernel = FECSIM_KERNEL_AUTO;
 };
 
+int pair_i8_shape(const DecoderDev& d);
+void pair_layout(const HdecHost& h, PairLayout* pl);
+hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why);
 void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);

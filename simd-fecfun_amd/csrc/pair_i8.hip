@@ -1,0 +1,326 @@
+// Row-pair int8 offset min-sum decoder for even Z, specialized per code shape
+// (layer count and row weights known at compile time; shifts and columns are read
+// from the decoder's table at run time).  Same results as the generic kernel
+// (decode_core.h), bit for bit; it only reorganizes the work for gfx950:
+//
+// * Rows r and r + Z/2 of a layer are handled by one lane as the two 16-bit halves of
+//   a VGPR, so every VALU op is a packed v_pk_* op on two rows.  For Z = 42 a
+//   codeword takes 21 lanes and a wave decodes 3 codewords.
+// * Posteriors live in LDS as int16 pairs D[c][t] = (L[c][t], L[c][(t + Z/2) mod Z]);
+//   each variable is stored twice (low half of pair t, high half of pair t - Z/2), so
+//   the lane reads its two rows' variables with one ds_read_b32 and no rotation fix-up,
+//   and writes the pair and its half-swapped twin.
+// * Pair t of column c sits at dword (c*CS + (t mod Z/2) + 32*(t >= Z/2)): a lane group
+//   reads Z/2 consecutive pairs (mod Z), whose t mod Z/2 are distinct, so they hit
+//   distinct banks; slot bases are offset by Z/2 banks so the three slots of a wave
+//   never collide either.
+// * C2V messages (packed pairs) and the per-lane LDS addresses of every edge stay in
+//   VGPRs for the whole decode.
+// Saturating int8 semantics: Q = sat8(L - R) via v_pk_sub + clamp, |Q| as
+// max(Q, -Q) in int16 (so abs(-128) = 128 and |Q| - beta never wraps for 1 <= beta),
+// R = (S xor sign Q) ? -M : M, L = sat8(Q + R).
+#include "decode_core.h"
+#include "pair_shapes.h"
+
+namespace fecsim {
+
+typedef short __attribute__((ext_vector_type(2))) short2v;
+typedef unsigned short __attribute__((ext_vector_type(2))) ushort2v;
+
+__device__ __forceinline__ uint32_t as_u(short2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ short2v as_s(uint32_t v) { return __builtin_bit_cast(short2v, v); }
+__device__ __forceinline__ ushort2v as_us(uint32_t v) { return __builtin_bit_cast(ushort2v, v); }
+__device__ __forceinline__ uint32_t as_u(ushort2v v) { return __builtin_bit_cast(uint32_t, v); }
+
+__device__ __forceinline__ short2v smax(short2v a, short2v b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ short2v smin(short2v a, short2v b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ ushort2v umin(ushort2v a, ushort2v b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ ushort2v umax(ushort2v a, ushort2v b) { return __builtin_elementwise_max(a, b); }
+
+struct PairArgs {
+  const uint32_t* adrtab;  // [half][ne]: per lane and edge, LDS byte offsets (in the slot) of the
+                           // pair read and of its twin, packed low | high << 16
+  int z, half, mcol;       // Z, Z/2
+  int cw_per_wave, waves;
+  int col_dwords;          // dwords per column in LDS
+  int slot_dwords;         // dwords per slot (stride)
+  uint32_t num_cw, max_iter;
+  long long chunk;         // codewords per wave (persistent grid)
+  int max_blocks;          // resident-grid size bound (0: one block per cw_per_wave*waves codewords)
+  int beta, cap;           // int8 beta (1..127) and cap (0..127)
+  const int8_t* llr_in;    // dense [num_cw][mcol*z]
+  int8_t* post_out;
+  uint32_t* iters;
+  int8_t* c2v;             // [num_cw][E] or null
+  unsigned long long* tot_iter;
+};
+
+template <class SH>
+__global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
+  constexpr int NE = SH::ne;
+  HIP_DYNAMIC_SHARED(uint32_t, smem);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L = a.half, Z = a.z, CPW = a.cw_per_wave;
+  const bool real = lane < CPW * L;          // lanes past the last slot mirror lane 0
+  const int slot = real ? lane / L : 0;
+  const int lr = real ? lane - slot * L : 0;
+  uint32_t* sbase = smem + (size_t)(wave * CPW + slot) * a.slot_dwords;
+
+  // per-edge LDS byte addresses of the pair this lane reads (t = (s + lr) mod Z) and of
+  // its twin ((t + Z/2) mod Z), both absolute in the block's LDS (< 64 KiB)
+  const uint32_t sb_off = (uint32_t)((wave * CPW + slot) * a.slot_dwords * 4);
+  const uint32_t sb2 = sb_off | (sb_off << 16);
+  uint32_t adr[NE];
+  const uint32_t* at = a.adrtab + (size_t)lr * NE;   // [lane][edge]: immediate offsets
+#pragma unroll
+  for (int e = 0; e < NE; ++e) adr[e] = at[e] + sb2;
+  char* sb = reinterpret_cast<char*>(smem);
+
+  // persistent work: this wave decodes codewords [w0, w1); each slot takes the next one
+  // as soon as its current codeword stops
+  const long long gw = (long long)blockIdx.x * a.waves + wave;
+  const long long w0 = gw * a.chunk;
+  long long w1 = w0 + a.chunk;
+  if (w1 > (long long)a.num_cw) w1 = a.num_cw;
+  long long next = w0;
+  long long cw = -1;                 // codeword of this lane's slot, -1 when idle
+  bool active = false;
+  uint32_t iter = 0;
+  uint32_t c2v[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) c2v[e] = 0u;
+  const short2v lo8 = short2v{-128, -128}, hi8 = short2v{127, 127};
+  const short2v beta2 = short2v{(short)a.beta, (short)a.beta};
+  const short2v cap2 = short2v{(short)a.cap, (short)a.cap};
+  const unsigned long long gmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
+  unsigned long long my_iters = 0;   // summed by lane 0 of each slot
+
+  for (;;) {
+    // ---- refill: idle slots take the next codewords of the wave's range ----
+    const unsigned long long want = __ballot(real && lr == 0 && !active);
+    if (want != 0ull && next < w1) {
+      const int rank = __popcll(want & ((1ull << (slot * L)) - 1ull));
+      const long long mine = next + rank;
+      const bool take = !active && mine < w1;
+      next += __popcll(want);
+      if (take) {
+        cw = mine;
+        active = true;
+        iter = 0;
+        if (real) {
+          const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
+          for (int c = 0; c < a.mcol; ++c) {
+            const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
+            const uint32_t p = (uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16);
+            sbase[c * a.col_dwords + lr] = p;
+            sbase[c * a.col_dwords + lr + 32] = (p >> 16) | (p << 16);
+          }
+        }
+      }
+      const uint32_t keep = take ? 0u : ~0u;   // fresh codeword: C2V messages start at zero
+#pragma unroll
+      for (int e = 0; e < NE; ++e) c2v[e] &= keep;
+    }
+    if (__ballot(active) == 0ull) break;
+    wave_sync();
+
+    // ---- one layered iteration; lanes of idle slots compute on their own (dead) slot ----
+#pragma unroll
+    for (int e = 0; e < NE; ++e) asm volatile("" : "+v"(adr[e]));   // unpack at each use, not hoisted
+    int e0 = 0;
+#pragma unroll
+    for (int l = 0; l < SH::mrow; ++l) {
+      constexpr int MAXW = SH::maxw;
+      const int w = SH::w[l];
+      // A'_j = max(|Q_j|, beta) = A_j + beta; min1/min2 are tracked on A' and beta is
+      // subtracted once per row.  The exclusive minimum is
+      //   M_j = min(cap, j is the argmin ? min2 : min1) = max(mv1, min(mv2, min1 + min2 - A_j))
+      // with mv = min(cap, min - beta): every A_j other than the argmin is >= min2, so
+      // min1 + min2 - A_j <= min1 there, and it equals min2 at the argmin (ties included).
+      short2v q[MAXW];
+      short2v A[MAXW];
+      uint32_t S = 0;
+      short2v m1 = short2v{255, 255}, m2 = short2v{255, 255};
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          const uint32_t x = *reinterpret_cast<const uint32_t*>(sb + (adr[e0 + j] & 0xffffu));
+          short2v qq = as_s(x) - as_s(c2v[e0 + j]);
+          qq = smin(smax(qq, lo8), hi8);
+          q[j] = qq;
+          const short2v av = smax(smax(qq, beta2), -qq);
+          A[j] = av;
+          S ^= as_u(qq);
+          m2 = smin(m2, smax(m1, av));
+          m1 = smin(m1, av);
+        }
+      }
+      const short2v mv1 = smin(cap2, m1 - beta2);
+      const short2v mv2 = smin(cap2, m2 - beta2);
+      const short2v sumb = m1 + m2 - beta2;
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          const short2v m = smax(mv1, smin(mv2, sumb - A[j]));   // j == argmin ? min2 : min1, capped
+          const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
+          const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
+          short2v g = q[j] + as_s(rn);
+          g = smin(smax(g, lo8), hi8);
+          c2v[e0 + j] = rn;
+          const uint32_t gu = as_u(g);
+          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] & 0xffffu)) = gu;
+          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] >> 16)) = __builtin_amdgcn_alignbit(gu, gu, 16);
+        }
+      }
+      e0 += w;
+      wave_sync();
+      __builtin_amdgcn_sched_barrier(0);   // schedule layer by layer: bounded live ranges
+    }
+    // ---- syndrome after the full iteration, LDPCdecode.cpp:172-195 ----
+#pragma unroll
+    for (int e = 0; e < NE; ++e) asm volatile("" : "+v"(adr[e]));   // no reuse of the layers' unpacked addresses
+    bool fail = false;
+    e0 = 0;
+#pragma unroll
+    for (int l = 0; l < SH::mrow; ++l) {
+      const int w = SH::w[l];
+      uint32_t par = 0;
+#pragma unroll
+      for (int j = 0; j < SH::maxw; ++j)
+        if (j < w) par ^= *reinterpret_cast<const uint32_t*>(sb + (adr[e0 + j] & 0xffffu));
+      fail |= (par & 0x80008000u) != 0u;
+      e0 += w;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const unsigned long long fb = __ballot(real && active && fail);
+    ++iter;
+    const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;
+    const bool finish = active && (!cwfail || iter == a.max_iter);   // LDPCdecode.cpp:192-197
+    if (finish && real) {
+      int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
+      for (int c = 0; c < a.mcol; ++c) {
+        const uint32_t p = sbase[c * a.col_dwords + lr];
+        g[c * Z + lr] = (int8_t)(p & 0xff);
+        g[c * Z + lr + L] = (int8_t)((p >> 16) & 0xff);
+      }
+      if (a.c2v) {
+        // opaque row offset: keeps the NE store offsets from being hoisted out of the
+        // decode loop as loop-invariant live registers
+        int zo = Z;
+        asm volatile("" : "+v"(zo));
+        int8_t* gc = a.c2v + cw * (long long)(NE * Z) + lr;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          gc[e * zo] = (int8_t)(c2v[e] & 0xff);
+          gc[e * zo + L] = (int8_t)((c2v[e] >> 16) & 0xff);
+        }
+      }
+      if (lr == 0) {
+        if (a.iters) a.iters[cw] = iter;
+        my_iters += iter;
+      }
+    }
+    active = active && !finish;
+  }
+  if (a.tot_iter) {
+    unsigned long long s = 0;
+    for (int q = 0; q < CPW; ++q) s += (unsigned long long)__shfl((long long)my_iters, q * L, 64);
+    if (lane == 0 && s) atomicAdd(a.tot_iter, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+int pair_i8_shape(const DecoderDev& d) {
+  const HdecHost& h = d.h;
+  if (d.elem != FECSIM_I8 || d.mode != FECSIM_MIN) return -1;
+  if (h.z % 2 != 0 || h.z / 2 > 32 || h.z / 2 < 2) return -1;
+  // one wave's slots must fit the 64 KiB block LDS
+  if ((64 / (h.z / 2)) * (h.mcol * (32 + h.z / 2) + 32) * 4 > 64 * 1024) return -1;
+  const float beta = d.params.beta_offset, mrm = d.params.max_row_metric;
+  const int b8 = (int)(int8_t)(int)beta;
+  if (mrm < 1e-6f) return -1;
+  const int c8 = (int)(int8_t)(int)mrm;
+  if (b8 < 1 || c8 < 0) return -1;
+  return pair_shape_index(h.mrow, h.row_weights.data());
+}
+
+template <class SH>
+static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
+  PairArgs a = in;
+  if (a.num_cw == 0) return hipSuccess;
+  const size_t lds = (size_t)a.waves * a.cw_per_wave * a.slot_dwords * 4;
+  // persistent grid: as many blocks as stay resident (x2 to smooth the tail), each wave
+  // working through a contiguous chunk of codewords with per-slot refill
+  static int resident = -1;
+  if (resident < 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH>, 64 * a.waves, lds) != hipSuccess)
+      return hipErrorInvalidValue;
+    resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+  }
+  const long long slots_per_block = (long long)a.waves * a.cw_per_wave;
+  long long blocks = (a.num_cw + slots_per_block - 1) / slots_per_block;
+  const long long cap_blocks = a.max_blocks > 0 ? a.max_blocks : 2LL * resident;
+  if (blocks > cap_blocks) blocks = cap_blocks;
+  const long long waves = blocks * a.waves;
+  a.chunk = (a.num_cw + waves - 1) / waves;
+  hipLaunchKernelGGL((k_pair_i8<SH>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
+  return hipGetLastError();
+}
+
+// Host-side layout of the pair kernel for decoder `d` (tables in d.h).
+void pair_layout(const HdecHost& h, PairLayout* pl) {
+  pl->half = h.z / 2;
+  pl->cw_per_wave = 64 / pl->half;
+  pl->col_dwords = 32 + pl->half;
+  // slot stride: >= mcol*col_dwords, and = half (mod 32) so slot bases land Z/2 banks apart
+  int sd = h.mcol * pl->col_dwords;
+  while (sd % 32 != pl->half % 32) ++sd;
+  pl->slot_dwords = sd;
+  const int wave_bytes = pl->cw_per_wave * sd * 4;
+  pl->waves = (64 * 1024) / wave_bytes;
+  if (pl->waves > 4) pl->waves = 4;
+  const int L = pl->half, Z = h.z;
+  pl->adrtab.assign((size_t)h.total_hc_ones * L, 0u);
+  for (int e = 0; e < h.total_hc_ones; ++e) {
+    const int c = h.row_active_cols[e], s = h.row_offset[e];
+    for (int lr = 0; lr < L; ++lr) {
+      const int t = (s + lr) % Z, tw = (t + L) % Z;
+      const int d0 = c * pl->col_dwords + (t >= L ? t - L + 32 : t);
+      const int d1 = c * pl->col_dwords + (tw >= L ? tw - L + 32 : tw);
+      pl->adrtab[(size_t)lr * h.total_hc_ones + e] = (uint32_t)(d0 * 4) | ((uint32_t)(d1 * 4) << 16);
+    }
+  }
+}
+
+hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t st) {
+  const HdecHost& h = d.h;
+  const PairLayout& pl = d.pair;
+  PairArgs a{};
+  a.adrtab = d.d_pair_adr;
+  a.z = h.z;
+  a.half = pl.half;
+  a.mcol = h.mcol;
+  a.cw_per_wave = pl.cw_per_wave;
+  a.col_dwords = pl.col_dwords;
+  a.slot_dwords = pl.slot_dwords;
+  a.waves = pl.waves;
+  a.num_cw = ka.num_cw;
+  a.max_iter = d.params.max_iter;
+  a.beta = (int)(int8_t)(int)d.params.beta_offset;
+  a.cap = (int)(int8_t)(int)d.params.max_row_metric;
+  a.llr_in = reinterpret_cast<const int8_t*>(ka.metrics_in);
+  a.post_out = reinterpret_cast<int8_t*>(ka.metrics_out);
+  a.iters = ka.iters;
+  a.c2v = reinterpret_cast<int8_t*>(ka.c2v);
+  a.tot_iter = ka.tot_iter;
+  return pair_dispatch(d.pair_shape, a, st);
+}
+
+#define FECSIM_PAIR_LAUNCH(SH) return launch_shape<SH>(a, st);
+#include "pair_shapes_dispatch.h"
+
+}  // namespace fecsim

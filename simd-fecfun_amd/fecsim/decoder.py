@@ -101,6 +101,14 @@ class Decoder:
         self.elem, self.mode, self.elem_bytes = e.value, m.value, b.value
         self.np_dtype = NP_OF_ELEM[self.elem]
 
+    def select_kernel(self, kernel: int) -> None:
+        """lib.KERNEL_AUTO / KERNEL_GENERIC / KERNEL_PAIR (results are identical)."""
+        _L.check(_L.lib().fecsim_decoder_select_kernel(self._d, kernel), "fecsim_decoder_select_kernel")
+
+    @property
+    def kernel(self) -> int:
+        return _L.lib().fecsim_decoder_kernel(self._d)
+
     @property
     def torch_dtype(self):
         import torch

@@ -1,0 +1,59 @@
+"""Generates csrc/pair_shapes.h and csrc/pair_shapes_dispatch.h: the code shapes (layer
+count and row weights) the row-pair int8 decoder (csrc/pair_i8.hip) is compiled for.
+
+A shape is the row-weight sequence of a base matrix; the shifts and columns stay run-time
+data, so one shape serves every lifting size, num_par and row relabeling of a code.
+Run by simd-fecfun_amd/build.py before compiling.
+"""
+from __future__ import annotations
+
+import os
+
+from . import codes
+
+# codes whose lifting size can be even and <= 64 (pair kernel: Z/2 <= 32 banks)
+PAIR_SETS = ["11ad 1/2", "11ad 13/16", "11ad 3/4", "11ad 5/8", "11n 1/2 z54", "11n 2/3 z54", "11n 3/4 z54",
+             "11n 1/2 z27", "16e 1/2", "16e 2/3A", "ITU G.hn 1/2 z14"]
+MAX_EDGES = 90
+
+
+def shapes():
+    out = []
+    for ms in PAIR_SETS:
+        hc, _, _ = codes.get_hmatrix(ms)
+        w = tuple(int((r >= 0).sum()) for r in hc)
+        if sum(w) <= MAX_EDGES and w not in [s for s, _ in out]:
+            out.append((w, ms))
+    return out
+
+
+def emit(csrc_dir: str) -> None:
+    sh = shapes()
+    h = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.", "#pragma once", "#include <stdint.h>", "",
+         "namespace fecsim {", ""]
+    for i, (w, ms) in enumerate(sh):
+        h.append("struct PairShape%d {  // %s" % (i, ms))
+        h.append("  static constexpr int mrow = %d, ne = %d, maxw = %d;" % (len(w), sum(w), max(w)))
+        h.append("  static constexpr int w[%d] = {%s};" % (len(w), ", ".join(map(str, w))))
+        h.append("};")
+    h += ["", "inline int pair_shape_index(int mrow, const int32_t* w) {"]
+    for i, (w, _) in enumerate(sh):
+        cond = " && ".join("w[%d] == %d" % (k, x) for k, x in enumerate(w))
+        h.append("  if (mrow == %d && %s) return %d;" % (len(w), cond, i))
+    h += ["  return -1;", "}", "", "struct PairArgs;",
+          "hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st);", "", "}  // namespace fecsim", ""]
+    d = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
+         "// included inside namespace fecsim by pair_i8.hip after FECSIM_PAIR_LAUNCH is defined",
+         "hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st) {", "  switch (shape) {"]
+    for i in range(len(sh)):
+        d.append("  case %d: FECSIM_PAIR_LAUNCH(PairShape%d)" % (i, i))
+    d += ["  default: return hipErrorInvalidValue;", "  }", "}", ""]
+    _write(os.path.join(csrc_dir, "pair_shapes.h"), "\n".join(h))
+    _write(os.path.join(csrc_dir, "pair_shapes_dispatch.h"), "\n".join(d))
+
+
+def _write(path, text):
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libfecsim.so")
 
 I8, I16, I32, F32, F64 = 0, 1, 2, 3, 4
 MIN, PROD = 0, 1
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_PAIR = 0, 1, 2
 
 # (name, restype, argtypes) of every exported entry point
 _vp, _u32, _i32, _int = C.c_void_p, C.c_uint32, C.c_int32, C.c_int
@@ -25,6 +26,8 @@ SIGNATURES = [
     ("fecsim_hdec_info", _int, [_vp, _vp]),
     ("fecsim_hdec_free", None, [_vp]),
     ("fecsim_decoder_create", _int, [_vp, C.c_char_p, _vp, C.POINTER(_vp)]),
+    ("fecsim_decoder_select_kernel", _int, [_vp, _int]),
+    ("fecsim_decoder_kernel", _int, [_vp]),
     ("fecsim_decoder_info", _int, [_vp, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
     ("fecsim_decoder_free", None, [_vp]),
     ("fecsim_decode_set", _int, [_vp, _u32, _int, _vp, _vp, _vp, _vp, _vp]),

@@ -170,3 +170,61 @@ def test_gpu_large_batch_properties(cuda):
     ref = O.decode_ideal(cs.dec, llr.cpu().numpy()[idx])
     assert np.array_equal(out.cpu().numpy()[idx], ref[0]) and np.array_equal(it[idx].astype(np.uint32), ref[2])
     _say("mean iterations at -1 dB:", it.mean())
+
+
+PAIR_CASES = [
+    # MatrixSet, num_par, z_over, snr, ncw, beta, cap
+    ("11ad 1/2", 1, None, -1.0, 1000, 1.0, 31.0),
+    ("11ad 1/2", 1, None, 0.0, 7, 1.0, 31.0),
+    ("11ad 1/2", 1, None, -0.5, 1, 2.0, 20.0),
+    ("11ad 13/16", 1, None, 3.0, 300, 1.0, 31.0),
+    ("11ad 3/4", 1, None, 2.0, 300, 1.0, 31.0),
+    ("11n 3/4 z54", 1, None, 2.0, 200, 1.0, 31.0),
+    ("11n 1/2 z27", 2, None, -0.5, 200, 1.0, 31.0),   # Z = 54
+    ("16e 1/2", 1, 48, -0.5, 200, 1.0, 31.0),
+    ("ITU G.hn 1/2 z14", 1, None, 0.0, 500, 1.0, 31.0),
+]
+
+
+@pytest.mark.parametrize("case", PAIR_CASES, ids=lambda c: "%s-np%d-z%s-%g-n%d" % c[:5])
+def test_gpu_pair_kernel_matches_oracle(case, cuda):
+    """The row-pair int8 kernel (even Z) against the oracle and the generic kernel:
+    posteriors, C2V, iteration counts and the summed iterations, bit for bit."""
+    import torch
+    from fecsim import lib as L
+    ms, npar, zo, snr, ncw, beta, cap = case
+    _say("pair", case)
+    cs = codes.gen_structs(ms, num_par=npar, simd_size=512, elem_size=8, z_over=zo)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, snr)
+    rng = np.random.default_rng(4)
+    llr[: max(1, ncw // 10)] = rng.integers(-128, 128, size=(max(1, ncw // 10), cs.n))   # saturation corners
+    ref = O.decode_ideal(cs.dec, llr, max_row_metric=cap, beta=beta)
+    h = Hdec.from_tables(cs.dec)
+    for kern in (L.KERNEL_PAIR, L.KERNEL_GENERIC):
+        d = Decoder(h, "Vec64c_min", 10, cap, beta)
+        d.select_kernel(kern)
+        assert d.kernel == kern
+        x = torch.from_numpy(llr).cuda()
+        out = torch.empty_like(x)
+        iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
+        tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+        d.decode(x, out, iters=iters, c2v=c2v, tot_iter=tot)
+        torch.cuda.synchronize()
+        assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2]), kern
+        assert np.array_equal(out.cpu().numpy(), ref[0]), kern
+        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1]), kern
+        assert int(tot.item()) == ref[3]
+
+
+def test_gpu_pair_kernel_selected_for_11ad(cuda):
+    from fecsim import lib as L
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    h = Hdec.from_tables(cs.dec)
+    assert Decoder(h, "Vec64c_min", 10, 31.0, 1.0).kernel == L.KERNEL_PAIR
+    assert Decoder(h, "Vec64c_min", 10, 31.0, 0.0).kernel == L.KERNEL_GENERIC   # beta 0: abs(-128) wrap path
+    assert Decoder(h, "Vec32s_min", 10, 31.0, 1.0).kernel == L.KERNEL_GENERIC
+    odd = codes.gen_structs("11n 1/2 z27", simd_size=512, elem_size=8)
+    assert Decoder(Hdec.from_tables(odd.dec), "Vec64c_min", 10, 31.0, 1.0).kernel == L.KERNEL_GENERIC
