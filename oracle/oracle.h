@@ -19,7 +19,7 @@
  *   orc_noise_llr           the product's counter-based noise scheme (splitmix64 +
  *                           the reference's Box-Muller transform), for bit-exact tests
  *
- * Parity pins: see tests/test_oracle_*.py.  Functions with no pin beyond their own
+ * Parity pins: tests/test_oracle.py and tests/test_codes.py.  Functions with no pin beyond their own
  * restatement are marked "parity unpinned" where they are declared.
  */
 #ifndef FECSIM_ORACLE_H
