@@ -125,6 +125,120 @@ int fecsim_decode(fecsim_decoder *d, uint32_t num_cw, int layout_p, const void *
 int fecsim_decode_set_host(fecsim_decoder *d, uint32_t num_cw, int layout_p, void *metrics_host,
                            uint32_t *iters_host, unsigned long long *tot_iter);
 
+/* ------------------------------------------------------------------------- */
+/* Encoder tables: LDPCencode.raw.dat                                         */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fecsim_henc fecsim_henc;
+
+typedef struct {
+    int32_t z, mrow, mcol, num_par;  /* header of the file (enc_struct_LDPC.h:12-28) */
+    int32_t punc_cnt, fill_cnt;      /* lengths of the puncture / filler range lists */
+    int32_t punc_bits, fill_bits;    /* bits those lists mark (fill_cnt of "Matrix Params") */
+    int32_t col_words;               /* 32-bit words per column of the dense bit layout, ceil(z/32) */
+    int32_t num_src;                 /* total rotated sources of the encoding schedule */
+} fecsim_henc_info_t;
+
+/* Replaces encode_matrix_allocate() (src/LDPCEnc/encode_matrix_allocate.cpp:38-92):
+ * same file format (gen_encode.m:83-84), puncture/filler lists turned into masks
+ * as enc_fill_mask() does (:13-36, ranges toggle bits).  Host-only; device copies
+ * of the tables are made on first use.  Needs z <= 2048. */
+int fecsim_henc_load(const char *path, fecsim_henc **out);
+int fecsim_henc_from_words(const int32_t *words, size_t nwords, fecsim_henc **out);
+int fecsim_henc_info(const fecsim_henc *h, fecsim_henc_info_t *info);
+/* masks as [mcol][col_words] uint32 (row r of column c at word c*col_words + r/32,
+ * bit r%32); either pointer may be NULL */
+int fecsim_henc_masks(const fecsim_henc *h, uint32_t *punct, uint32_t *fill);
+void fecsim_henc_free(fecsim_henc *h);
+
+/* Codeword bits in device memory are [num_cw][mcol][col_words] uint32 words, row r of
+ * a column at word r/32, bit r%32, data columns first (col_words >= ceil(z/32)).
+ * col_words = 16 is the reference's cw_bits layout (one 512-bit ENC_CLASS word per
+ * column, enc_defs.h). */
+
+/* Replaces gen_rand_databits() (src/Top/gen_rand_databits.cpp:11-40) + LDPCencode()
+ * (src/LDPCEnc/LDPCencode.cpp:40-100): fills the data columns of codewords
+ * cw0 .. cw0+num_cw-1 with counter-based random bits (seed, codeword index; masked to
+ * z bits and to non-filler bits) and computes the parity columns.  The stream is
+ * this library's own (splitmix64 per 64-bit chunk), not the reference's xoshiro
+ * stream, whose state depends on every earlier draw. */
+int fecsim_gen_codewords(fecsim_henc *h, uint64_t seed, uint64_t cw0, uint32_t num_cw, int col_words,
+                         uint32_t *bits, void *stream);
+/* Replaces LDPCencode() alone: reads the data columns of bits (masked to z bits) and
+ * writes the parity columns. */
+int fecsim_encode(fecsim_henc *h, uint32_t num_cw, int col_words, uint32_t *bits, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Channel: BPSK + AWGN -> quantized LLRs                                     */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    float snr_db;       /* Es/N0 in dB: sigma = 10^(-(snr + 10 log10 2)/20) (gen_metrics.cpp:28-30) */
+    float llr_scale;    /* LLR_scaling_factor: LLR = 2*scale/sigma^2 * y */
+    float max_abs_llr;  /* clamp; filler bits get +max_abs_llr, punctured bits 0 */
+} fecsim_chan_params;
+
+/* Replaces normdist2() + gen_metrics() (src/RNGfunc/prng_functions.cpp:148-186,
+ * src/Top/gen_metrics.cpp:18-121): for codewords cw0.. of `bits`, noise from the
+ * counter-based generator keyed by (seed, codeword, column pair, row), the
+ * reference's Box-Muller transform (fastm2log, sincos_norm), y = fma(n, sigma, +-1),
+ * LLR clamped, punctured -> 0, filler -> +max, integers rounded to nearest even and
+ * truncated to the element width.  Writes `elem` LLRs in layout_p's layout (as
+ * fecsim_decode_set) and adds the count of sign flips of y (the uncoded bit errors)
+ * to *uncoded_err (device uint64, may be NULL).  The same seed and codeword index
+ * give the same noise at every SNR, as the reference reuses its noise buffer
+ * across SNR points (LDPCtop.cpp:90-97). */
+int fecsim_gen_llr(fecsim_henc *h, const fecsim_chan_params *p, uint64_t seed, uint64_t cw0, uint32_t num_cw,
+                   int col_words, const uint32_t *bits, int elem, int layout_p, void *llr,
+                   unsigned long long *uncoded_err, void *stream);
+
+/* sigma and LLR factor the channel uses for p (host computation, gen_metrics.cpp:28-30) */
+int fecsim_chan_sigma(const fecsim_chan_params *p, float *sigma, float *factor);
+
+/* Replaces LDPC_count_errors() (src/Top/LDPC_count_errors.cpp:40-136): hard decisions
+ * (value < 0) of the data columns of `post` against `bits`; adds the bit errors to
+ * counts[0] and the codeword errors (one per interleaved codeword slot r % num_par
+ * with an error) to counts[1] (device uint64[2]).  cw_bit_err (device [num_cw] or
+ * NULL) receives each codeword's bit errors. */
+int fecsim_count_errors(fecsim_henc *h, uint32_t num_cw, int col_words, const uint32_t *bits, int elem,
+                        int layout_p, const void *post, unsigned long long *counts, uint32_t *cw_bit_err,
+                        void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Simulation loop                                                            */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fecsim_sim fecsim_sim;
+
+typedef struct {
+    uint32_t num_cw;        /* codewords per inner loop (sim_struct.num_codeword) */
+    uint32_t num_snr;       /* SNR points */
+    const float *snr_db;    /* [num_snr], Es/N0 dB as gen_metrics */
+    float llr_scale, max_abs_llr;
+    uint64_t seed;
+    int use_graph;          /* 1: capture one inner loop in a HIP graph and replay it */
+} fecsim_sim_params;
+
+/* per SNR point, accumulated since the last reset: the fields of the reference's
+ * "loop" line (LDPCtop.cpp:127-129) */
+typedef struct {
+    unsigned long long cw, uncoded_bit_err, iters, bit_err, cw_err;
+} fecsim_sim_stats;
+
+/* Replaces the driver loop of src/Top/LDPCtop.cpp:86-121 for one decoder: every inner
+ * loop generates num_cw new codewords (data bits + encoder), then for every SNR
+ * point generates their LLRs (same noise at every point), decodes them in place
+ * and counts errors, all on the device.  The decoder's element type sets the LLR
+ * type; the layout is dense.  Owns its device buffers. */
+int fecsim_sim_create(fecsim_henc *h, fecsim_decoder *d, const fecsim_sim_params *p, fecsim_sim **out);
+/* enqueues `loops` inner loops on stream (asynchronous) */
+int fecsim_sim_run(fecsim_sim *s, uint32_t loops, void *stream);
+/* synchronises the device and copies the counters ([num_snr]) */
+int fecsim_sim_stats_get(fecsim_sim *s, fecsim_sim_stats *out);
+/* zeroes the counters (stream-ordered); the codeword index keeps advancing */
+int fecsim_sim_reset(fecsim_sim *s, void *stream);
+void fecsim_sim_free(fecsim_sim *s);
+
 #ifdef __cplusplus
 }
 #endif

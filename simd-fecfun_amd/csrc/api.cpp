@@ -17,11 +17,7 @@ struct fecsim_hdec {
   int32_t len_f2b = 0, len_b2f = 0;
 };
 
-struct fecsim_decoder {
-  DecoderDev d;
-};
-
-namespace {
+namespace fecsim {
 
 thread_local std::string g_err;
 
@@ -33,6 +29,12 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
   return fail(FECSIM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+const int kElemBytes[5] = {1, 2, 4, 4, 8};
+
+}  // namespace fecsim
+
+namespace {
 
 // Parse the int32 words of LDPCdecode.raw.dat (decode_matrix_allocate.cpp:22-50)
 int parse_hdec(const int32_t* w, size_t n, fecsim_hdec* out) {
@@ -112,8 +114,6 @@ int parse_kind(const char* kind, int* elem, int* mode) {
   return FECSIM_OK;
 }
 
-const int kElemBytes[5] = {1, 2, 4, 4, 8};
-
 }  // namespace
 
 extern "C" {
@@ -133,7 +133,7 @@ const char* fecsim_strerror(int s) {
   }
 }
 
-const char* fecsim_last_error(void) { return g_err.c_str(); }
+const char* fecsim_last_error(void) { return fecsim::g_err.c_str(); }
 
 int fecsim_hdec_from_words(const int32_t* words, size_t nwords, fecsim_hdec** out) {
   if (!words || !out) return fail(FECSIM_E_INVAL, "null argument");
@@ -269,16 +269,7 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   a.num_cw = num_cw;
   a.metrics_in = llr_in;
   a.metrics_out = post_out;
-  if (layout_p == 0) {
-    a.cw_stride = a.core.n;
-    a.col_stride = h.z;
-    a.base = 0;
-  } else {
-    const long long spz = (h.z + layout_p - 1) / layout_p;
-    a.col_stride = (spz + 2) * layout_p;
-    a.cw_stride = a.col_stride * h.mcol;
-    a.base = layout_p;
-  }
+  layout_strides(h.z, h.mcol, layout_p, &a.cw_stride, &a.col_stride, &a.base);
   a.iters = iters;
   a.c2v = c2v;
   a.tot_iter = tot_iter;

@@ -21,10 +21,10 @@ struct HdecHost {
 struct HencHost {
   int32_t z = 0, punc_cnt = 0, fill_cnt = 0, num_par = 1, mrow = 0, mcol = 0;
   std::vector<int32_t> par_cnts, par_colnum, par_offset, punc_list, fill_list;
-  std::vector<int32_t> src_col, src_shift;       // per schedule entry
-  std::vector<uint32_t> punct_words, fill_words; // [mcol][wpc] 32-bit masks, bit r of column at word r>>5
-  int wpc = 0;                                    // 32-bit words per column = ceil(z/32)
-  int fill_bits = 0;                              // number of filler bits (fill_cnt of "Matrix Params")
+  std::vector<uint32_t> src;                      // per schedule entry: (column << 16) | shift
+  std::vector<uint32_t> punct_words, fill_words;  // [mcol][zw] masks, row r of a column at word r>>5
+  int zw = 0;                                     // 32-bit words per column = ceil(z/32)
+  int punc_bits = 0, fill_bits = 0;               // bits set in the masks
 };
 
 // Launch shape of the layered decoder: each wave owns `cw_per_wave` codeword
@@ -60,30 +60,52 @@ struct KernelArgs {
   unsigned long long* tot_iter;
 };
 
-// Fused simulation step: data bits -> encoder -> BPSK + AWGN -> quantized LLRs ->
-// decoder -> error counts, all inside one wave's LDS slice per codeword.
-struct SimArgs {
-  DecCore core;
-  // encoder
-  const int32_t* par_cnts;     // [mrow]
-  const int32_t* par_colnum;   // [mrow]
-  const int32_t* src_col;      // [nsrc]
-  const int32_t* src_shift;    // [nsrc]
-  const uint32_t* punct;       // [mcol][wpc]
-  const uint32_t* fill;        // [mcol][wpc]
-  int wpc, dwords;             // words per column; words per doubled column (2*wpc + 1)
-  int enc_bytes;               // LDS bytes per slot for the doubled columns
-  int num_par;                 // interleaved codewords per super codeword (error slots)
-  int ndata;                   // data columns
-  // channel
+// Data bits + encoder (sim.hip, k_encode)
+struct EncArgs {
+  const int32_t* cnt;        // [mrow] sources per parity column
+  const int32_t* colnum;     // [mrow] parity column index (among the parity columns)
+  const uint32_t* src;       // [nsrc] (source column << 16) | shift
+  const uint32_t* fill;      // [mcol][zw] filler mask words
+  int z, zw, mrow, mcol, ndata;
+  int cpw, slot_words, waves;  // codewords per wave, LDS words per codeword, waves per block
+  int col_words;             // words per column in the caller's bit buffer (>= zw)
+  int gen;                   // 1: generate the data columns, 0: read them from bits
   uint64_t seed;
-  unsigned long long cw0;      // global index of the first codeword
+  unsigned long long cw0;    // index of the first codeword (+ *cw0_dev when given)
+  const unsigned long long* cw0_dev;
+  uint64_t num_cw;
+  uint32_t* bits;            // [num_cw][mcol][col_words]
+};
+
+// Noise + quantized LLRs (sim.hip, k_llr)
+struct LlrArgs {
+  const uint32_t* bits;      // [num_cw][mcol][col_words]
+  const uint32_t* punct;     // [mcol][zw]
+  const uint32_t* fill;      // [mcol][zw]
+  int z, zw, mcol, hm, col_words;   // hm = ceil(mcol / 2) column pairs
+  uint64_t seed;
+  unsigned long long cw0;
+  const unsigned long long* cw0_dev;
+  uint64_t cw_off;           // first codeword of this launch within the batch
   uint32_t num_cw;
-  const float* sigma;          // [nsnr]
-  const float* factor;         // [nsnr]
-  float max_abs;
-  int nsnr;
-  unsigned long long* counters;   // [nsnr][FECSIM_NCOUNT]
+  float sigma, factor, max_abs;
+  void* out;                 // LLRs of the decoder's element type
+  long long cw_stride, col_stride, base;  // output layout in elements
+  unsigned long long* uncoded;            // device counter (added to), may be null
+};
+
+// Error counting (sim.hip, k_count)
+constexpr int kCountMaxTile = 64;
+struct CountArgs {
+  const uint32_t* bits;
+  int col_words, z, mcol, ndata, num_par;
+  int cpb;                   // codewords per block tile (set by the launcher)
+  uint64_t num_cw;
+  const void* post;
+  long long cw_stride, col_stride, base;
+  unsigned long long* counts;      // device [2]: bit errors, codeword errors (added to)
+  uint32_t* cw_bit_err;            // device [num_cw] or null
+  unsigned long long* cw_counter;  // device counter of codewords (added to), or null
 };
 
 // Layout of the row-pair int8 kernel (pair_i8.hip)
@@ -111,6 +133,36 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t 
 int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why);
 void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
-hipError_t launch_sim(const DecoderDev& d, const SimArgs& a, hipStream_t stream);
+int encode_shape(int z, int mcol, EncArgs* a);
+hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
+hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);
+hipError_t launch_count(int elem, const CountArgs& a, hipStream_t stream);
+hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t stream);
+hipError_t launch_advance(unsigned long long* p, unsigned long long by, hipStream_t stream);
+
+// error reporting shared by the C ABI translation units (api.cpp)
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+extern const int kElemBytes[5];
+
+// element strides of a metrics buffer: dense (layout_p = 0) or the reference's
+// padded P-lane layout [cw][mcol][(ceil(z/P)+2)*P] with data at +P
+inline void layout_strides(int z, int mcol, int layout_p, long long* cw_stride, long long* col_stride,
+                           long long* base) {
+  if (layout_p == 0) {
+    *col_stride = z;
+    *base = 0;
+  } else {
+    const long long spz = (z + layout_p - 1) / layout_p;
+    *col_stride = (spz + 2) * layout_p;
+    *base = layout_p;
+  }
+  *cw_stride = *col_stride * mcol;
+}
 
 }  // namespace fecsim
+
+struct fecsim_decoder {
+  fecsim::DecoderDev d;
+};

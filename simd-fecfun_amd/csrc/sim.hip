@@ -1,0 +1,383 @@
+// Simulation kernels around the decoder: counter-based data bits, the QC encoder,
+// BPSK + AWGN -> quantized channel LLRs, and error counting.
+//
+//   k_encode  src/Top/gen_rand_databits.cpp:11-40 + src/LDPCEnc/LDPCencode.cpp:40-100
+//   k_llr     src/RNGfunc/prng_functions.cpp:148-186 (normdist2) + src/Top/gen_metrics.cpp:18-121
+//   k_count   src/Top/LDPC_count_errors.cpp:40-136
+//
+// Bits of a codeword are [mcol][col_words] 32-bit words, row r of a column at word
+// r >> 5, bit r & 31 (with col_words = 16 this is byte-for-byte the reference's
+// 512-bit cw_bits word per column, bit r at byte r >> 3, bit r & 7).
+//
+// Random numbers are counter based, so every codeword can be generated by any
+// thread and a batch split over launches (or GPUs) reproduces the same stream:
+//   data word:   splitmix64((seed ^ DATA_KEY) + (q + 1) * GAMMA), q = (cw*mcol + c)*8 + k
+//                (64-bit chunk k of data column c)
+//   noise pair:  splitmix64(seed + (p + 1) * GAMMA), p = (cw*ceil(mcol/2) + col/2)*z + row,
+//                turned into two normal samples by the reference's Box-Muller
+//                transform (sin for even columns, cos for odd columns).
+#include <hip/hip_runtime.h>
+
+#include "decode_core.h"
+#include "fecmath.h"
+#include "internal.h"
+
+namespace fecsim {
+
+namespace {
+
+constexpr uint64_t kGamma = 0x9e3779b97f4a7c15ull;
+constexpr uint64_t kDataKey = 0xd1b54a32d192ed03ull;
+constexpr uint64_t kExtraKey = 0x5851f42d4c957f2dull;
+
+__device__ __forceinline__ uint32_t zmask_word(int z, int w) {
+  const int valid = z - 32 * w;
+  return valid >= 32 ? 0xffffffffu : (valid <= 0 ? 0u : (0xffffffffu >> (32 - valid)));
+}
+
+// 32 bits of a Z-bit column starting at row a (0 <= a < z), wrapping modulo z.
+// col[] holds the column with bits >= z cleared and one zero word after it.
+__device__ __forceinline__ uint32_t cyc32(const uint32_t* col, int z, int a) {
+  if (z >= 32) {
+    const int i = a >> 5;
+    const uint64_t pair = ((uint64_t)col[i + 1] << 32) | col[i];
+    uint32_t x = (uint32_t)(pair >> (a & 31));
+    const int rem = z - a;
+    if (rem < 32) x |= col[0] << rem;
+    return x;
+  }
+  const uint32_t c0 = col[0];
+  uint32_t x = 0;
+  int p = a;
+  for (int t = 0; t < 32; ++t) {
+    x |= ((c0 >> p) & 1u) << t;
+    if (++p == z) p = 0;
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// data bits + encoder: one wave holds `cpw` codewords, `zw` lanes per codeword
+// (lane w owns word w of every column); the columns live in the wave's LDS slice
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_encode(EncArgs a) {
+  extern __shared__ uint32_t lds[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int zw = a.zw, cstride = zw + 1;
+  const int slot = lane / zw, w = lane - slot * zw;
+  const bool lane_ok = slot < a.cpw;
+  uint32_t* col0 = lds + (size_t)wv * a.cpw * a.slot_words + (size_t)(lane_ok ? slot : 0) * a.slot_words;
+  const uint32_t zm = zmask_word(a.z, w);
+  const unsigned long long cw_base = a.cw0 + (a.cw0_dev ? *a.cw0_dev : 0ull);
+  const uint64_t dkey = a.seed ^ kDataKey;
+  const uint64_t step = (uint64_t)gridDim.x * a.waves * a.cpw;
+  for (uint64_t first = ((uint64_t)blockIdx.x * a.waves + wv) * a.cpw; first < a.num_cw; first += step) {
+    const uint64_t cw = first + slot;
+    const bool valid = lane_ok && cw < a.num_cw;
+    uint32_t* g = a.bits + (valid ? cw : 0) * (uint64_t)a.mcol * a.col_words;
+    // data columns
+    for (int c = 0; c < a.ndata; ++c) {
+      uint32_t v;
+      if (a.gen) {
+        const uint64_t q = ((uint64_t)(cw_base + cw) * (uint64_t)a.mcol + (uint64_t)c) * 8u + (uint64_t)(w >> 1);
+        const uint64_t r = fecmath::splitmix64(dkey + (q + 1) * kGamma);
+        v = (w & 1) ? (uint32_t)(r >> 32) : (uint32_t)r;
+        v &= zm & ~a.fill[c * zw + w];
+      } else {
+        v = valid ? g[(size_t)c * a.col_words + w] & zm : 0u;
+      }
+      if (lane_ok) {
+        col0[c * cstride + w] = v;
+        if (w == 0) col0[c * cstride + zw] = 0u;
+      }
+      if (a.gen && valid)
+        for (int x = w; x < a.col_words; x += zw) g[(size_t)c * a.col_words + x] = x < zw ? v : 0u;
+    }
+    wave_sync();
+    // parity columns in schedule order: XOR of rotated columns
+    int idx = 0;
+    for (int i = 0; i < a.mrow; ++i) {
+      uint32_t acc = 0;
+      const int cnt = a.cnt[i];
+      for (int k = 0; k < cnt; ++k) {
+        const uint32_t s = a.src[idx + k];
+        const int c = (int)(s >> 16), sh = (int)(s & 0xffffu);
+        int at = 32 * w + sh;
+        if (at >= a.z) at -= a.z;
+        acc ^= cyc32(col0 + c * cstride, a.z, at);
+      }
+      idx += cnt;
+      acc &= zm;
+      const int pc = a.ndata + a.colnum[i];
+      if (lane_ok) {
+        col0[pc * cstride + w] = acc;
+        if (w == 0) col0[pc * cstride + zw] = 0u;
+      }
+      wave_sync();
+    }
+    if (valid) {
+      for (int pc = a.ndata; pc < a.mcol; ++pc) {
+        const uint32_t v = col0[pc * cstride + w];
+        for (int x = w; x < a.col_words; x += zw) g[(size_t)pc * a.col_words + x] = x < zw ? v : 0u;
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// noise + quantized LLRs: one thread per (codeword, column pair, row)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void noise_pair(uint64_t seed, uint64_t p, float* s_out, float* c_out) {
+  // u = float((r & 0x7fffffff) ^ 0x7fffffff) * 2^-31, an extra draw scaled by 2^-62
+  // (plus 2^-63) when the top 28 bits are zero, g = sqrt(-2 ln u), angle = int32 * 2^-31
+  // (prng_functions.cpp:148-186, one pair per counter)
+  const uint64_t r = fecmath::splitmix64(seed + (p + 1) * kGamma);
+  const int32_t lo = (int32_t)(uint32_t)r, hi = (int32_t)(r >> 32);
+  const int32_t v = (lo & 0x7fffffff) ^ 0x7fffffff;
+  float u = (float)v * 0x1.0p-31f;
+  if ((v & 0x7ffffff8) == 0) {
+    const uint64_t r2 = fecmath::splitmix64((seed ^ kExtraKey) + (p + 1) * kGamma);
+    const int32_t v2 = ((int32_t)(uint32_t)r2 & 0x7fffffff) ^ 0x7fffffff;
+    u = u + __builtin_fmaf((float)v2, 0x1.0p-62f, 0x1.0p-63f);
+  }
+  const float g = __builtin_sqrtf(fecmath::fastm2log(u));
+  float s, c;
+  fecmath::sincos_norm((float)hi * 0x1.0p-31f, &s, &c);
+  *s_out = s * g;
+  *c_out = c * g;
+}
+
+// cvtps2dq: nearest-even, out of range or NaN -> INT32_MIN (gen_metrics.cpp:98 roundi)
+__device__ __forceinline__ int32_t roundi_x86(float x) {
+  if (!(fabsf(x) < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)__builtin_rintf(x);
+}
+
+template <typename T>
+__device__ __forceinline__ T to_elem(float l) {
+  if constexpr (std::is_floating_point<T>::value) return (T)l;
+  else return (T)roundi_x86(l);   // truncation to the element width, as compress() does
+}
+
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* red) {
+  // wave sum with DPP-free shuffles, then across the block's waves
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+  return t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
+  __shared__ unsigned long long red[4];
+  const uint32_t per_cw = (uint32_t)a.hm * (uint32_t)a.z;
+  const unsigned long long cw_base = a.cw0 + (a.cw0_dev ? *a.cw0_dev : 0ull);
+  T* out = reinterpret_cast<T*>(a.out);
+  unsigned long long err = 0;
+  const uint32_t total = a.num_cw * per_cw;   // host splits launches below 2^31
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t cw = t / per_cw;
+    const uint32_t rem = t - cw * per_cw;
+    const uint32_t k = rem / (uint32_t)a.z;
+    const int r = (int)(rem - k * (uint32_t)a.z);
+    const uint64_t p = ((uint64_t)(cw_base + a.cw_off + cw) * (uint64_t)a.hm + k) * (uint64_t)a.z + (uint64_t)r;
+    float ns, nc;
+    noise_pair(a.seed, p, &ns, &nc);
+    const uint64_t gcw = a.cw_off + cw;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int col = 2 * (int)k + h;
+      if (col >= a.mcol) break;
+      const uint32_t word = a.bits[(gcw * a.mcol + col) * a.col_words + (r >> 5)];
+      const uint32_t bit = (word >> (r & 31)) & 1u;
+      const float orig = bit ? -1.0f : 1.0f;
+      const float rx = __builtin_fmaf(h ? nc : ns, a.sigma, orig);
+      err += (__float_as_uint(orig) ^ __float_as_uint(rx)) >> 31;
+      float l = rx * a.factor;
+      l = a.max_abs < l ? a.max_abs : l;
+      l = -a.max_abs > l ? -a.max_abs : l;
+      const int mw = col * a.zw + (r >> 5);
+      if ((a.punct[mw] >> (r & 31)) & 1u) l = 0.0f;
+      if ((a.fill[mw] >> (r & 31)) & 1u) l = a.max_abs;
+      out[(long long)gcw * a.cw_stride + (long long)col * a.col_stride + a.base + r] = to_elem<T>(l);
+    }
+  }
+  const unsigned long long s = block_sum(err, red);
+  if (threadIdx.x == 0 && s && a.uncoded) atomicAdd(a.uncoded, s);
+}
+
+// ---------------------------------------------------------------------------
+// error counting: a block takes a tile of `cpb` codewords and walks their data
+// bits flat; errors are rare, so they are tallied with LDS atomics
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_count(CountArgs a) {
+  __shared__ uint32_t s_cnt[kCountMaxTile];
+  __shared__ unsigned long long s_mask[kCountMaxTile];
+  __shared__ unsigned long long red[4];
+  const T* post = reinterpret_cast<const T*>(a.post);
+  const uint32_t dz = (uint32_t)a.ndata * (uint32_t)a.z;
+  const uint32_t items = (uint32_t)a.cpb * dz;
+  for (int i = threadIdx.x; i < a.cpb; i += blockDim.x) { s_cnt[i] = 0; s_mask[i] = 0; }
+  __syncthreads();
+  unsigned long long bit_err = 0, cw_err = 0;
+  const uint64_t ntiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t cw_first = tile * a.cpb;
+    for (uint32_t f = threadIdx.x; f < items; f += blockDim.x) {
+      const uint32_t lc = f / dz;
+      const uint32_t rem = f - lc * dz;
+      const uint32_t c = rem / (uint32_t)a.z;
+      const int r = (int)(rem - c * (uint32_t)a.z);
+      const uint64_t cw = cw_first + lc;
+      if (cw >= a.num_cw) continue;
+      const T v = post[(long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base + r];
+      const uint32_t hard = v < T(0) ? 1u : 0u;
+      const uint32_t bit = (a.bits[(cw * a.mcol + c) * a.col_words + (r >> 5)] >> (r & 31)) & 1u;
+      if (hard != bit) {
+        atomicAdd(&s_cnt[lc], 1u);
+        atomicOr(&s_mask[lc], 1ull << (r % a.num_par));
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.cpb; i += blockDim.x) {
+      const uint64_t cw = cw_first + i;
+      const uint32_t n = s_cnt[i];
+      if (cw < a.num_cw) {
+        if (a.cw_bit_err) a.cw_bit_err[cw] = n;
+        bit_err += n;
+        cw_err += (unsigned long long)__popcll(s_mask[i]);
+      }
+      s_cnt[i] = 0;
+      s_mask[i] = 0;
+    }
+    __syncthreads();
+  }
+  const unsigned long long sb = block_sum(bit_err, red);
+  const unsigned long long sc = block_sum(cw_err, red);
+  if (threadIdx.x == 0) {
+    if (sb) atomicAdd(&a.counts[0], sb);
+    if (sc) atomicAdd(&a.counts[1], sc);
+    if (blockIdx.x == 0 && a.cw_counter) atomicAdd(a.cw_counter, (unsigned long long)a.num_cw);
+  }
+}
+
+// counters and the codeword offset of a simulation loop (fill and advance kernels
+// rather than memset nodes, so the loop can be captured in a graph)
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_advance(unsigned long long* p, unsigned long long by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += by;
+}
+
+int resident_blocks(const void* fn, int block, size_t lds) {
+  int dev = 0, per_cu = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  return per_cu * cus;
+}
+
+}  // namespace
+
+int encode_shape(int z, int mcol, EncArgs* a) {
+  a->zw = (z + 31) / 32;
+  if (a->zw > 64) return FECSIM_E_UNSUPPORTED;
+  a->cpw = 64 / a->zw;
+  a->slot_words = mcol * (a->zw + 1);
+  const int wave_bytes = a->cpw * a->slot_words * 4;
+  if (wave_bytes > 65536) {
+    a->cpw = 65536 / (a->slot_words * 4);
+    if (a->cpw < 1) return FECSIM_E_UNSUPPORTED;
+  }
+  a->waves = 65536 / (a->cpw * a->slot_words * 4);
+  if (a->waves > 4) a->waves = 4;
+  return FECSIM_OK;
+}
+
+hipError_t launch_encode(const EncArgs& a, hipStream_t st) {
+  if (a.num_cw == 0) return hipSuccess;
+  const size_t lds = (size_t)a.waves * a.cpw * a.slot_words * 4;
+  const uint64_t per_block = (uint64_t)a.waves * a.cpw;
+  uint64_t grid = (a.num_cw + per_block - 1) / per_block;
+  const uint64_t cap = (uint64_t)resident_blocks((const void*)k_encode, a.waves * 64, lds) * 4;
+  if (grid > cap) grid = cap;
+  hipLaunchKernelGGL(k_encode, dim3((uint32_t)grid), dim3(a.waves * 64), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_llr(int elem, const LlrArgs& a0, hipStream_t st) {
+  const uint64_t per_cw = (uint64_t)a0.hm * a0.z;
+  const uint32_t chunk = (uint32_t)((1ull << 30) / per_cw);
+  LlrArgs a = a0;
+  for (uint64_t done = 0; done < a0.num_cw; done += chunk) {
+    const uint32_t n = (uint32_t)((a0.num_cw - done) < chunk ? (a0.num_cw - done) : chunk);
+    a.cw_off = a0.cw_off + done;
+    a.num_cw = n;
+    const uint64_t threads = (uint64_t)n * per_cw;
+    uint64_t grid = (threads + 255) / 256;
+    const void* fn;
+    switch (elem) {
+    case FECSIM_I8: fn = (const void*)k_llr<int8_t>; break;
+    case FECSIM_I16: fn = (const void*)k_llr<int16_t>; break;
+    case FECSIM_I32: fn = (const void*)k_llr<int32_t>; break;
+    case FECSIM_F32: fn = (const void*)k_llr<float>; break;
+    default: fn = (const void*)k_llr<double>; break;
+    }
+    const uint64_t cap = (uint64_t)resident_blocks(fn, 256, 0) * 8;
+    if (grid > cap) grid = cap;
+    switch (elem) {
+    case FECSIM_I8: hipLaunchKernelGGL(k_llr<int8_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+    case FECSIM_I16: hipLaunchKernelGGL(k_llr<int16_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+    case FECSIM_I32: hipLaunchKernelGGL(k_llr<int32_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+    case FECSIM_F32: hipLaunchKernelGGL(k_llr<float>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL(k_llr<double>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_count(int elem, const CountArgs& a0, hipStream_t st) {
+  if (a0.num_cw == 0) return hipSuccess;
+  CountArgs a = a0;
+  const uint32_t dz = (uint32_t)a.ndata * a.z;
+  a.cpb = (int)(2048 / dz);
+  if (a.cpb < 1) a.cpb = 1;
+  if (a.cpb > kCountMaxTile) a.cpb = kCountMaxTile;
+  const uint64_t tiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
+  uint64_t grid = tiles;
+  const uint64_t cap = (uint64_t)resident_blocks((const void*)k_count<int8_t>, 256, 0) * 4;
+  if (grid > cap) grid = cap;
+  switch (elem) {
+  case FECSIM_I8: hipLaunchKernelGGL(k_count<int8_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+  case FECSIM_I16: hipLaunchKernelGGL(k_count<int16_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+  case FECSIM_I32: hipLaunchKernelGGL(k_count<int32_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+  case FECSIM_F32: hipLaunchKernelGGL(k_count<float>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+  default: hipLaunchKernelGGL(k_count<double>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(unsigned long long* p, unsigned long long by, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, p, by);
+  return hipGetLastError();
+}
+
+}  // namespace fecsim

@@ -16,7 +16,7 @@ MIN, PROD = 0, 1
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_PAIR = 0, 1, 2
 
 # (name, restype, argtypes) of every exported entry point
-_vp, _u32, _i32, _int = C.c_void_p, C.c_uint32, C.c_int32, C.c_int
+_vp, _u32, _i32, _int, _u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_int, C.c_uint64
 SIGNATURES = [
     ("fecsim_abi_version", _int, []),
     ("fecsim_strerror", C.c_char_p, [_int]),
@@ -33,6 +33,21 @@ SIGNATURES = [
     ("fecsim_decode_set", _int, [_vp, _u32, _int, _vp, _vp, _vp, _vp, _vp]),
     ("fecsim_decode", _int, [_vp, _u32, _int, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("fecsim_decode_set_host", _int, [_vp, _u32, _int, _vp, _vp, C.POINTER(C.c_ulonglong)]),
+    ("fecsim_henc_load", _int, [C.c_char_p, C.POINTER(_vp)]),
+    ("fecsim_henc_from_words", _int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    ("fecsim_henc_info", _int, [_vp, _vp]),
+    ("fecsim_henc_masks", _int, [_vp, _vp, _vp]),
+    ("fecsim_henc_free", None, [_vp]),
+    ("fecsim_gen_codewords", _int, [_vp, _u64, _u64, _u32, _int, _vp, _vp]),
+    ("fecsim_encode", _int, [_vp, _u32, _int, _vp, _vp]),
+    ("fecsim_gen_llr", _int, [_vp, _vp, _u64, _u64, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),
+    ("fecsim_chan_sigma", _int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("fecsim_count_errors", _int, [_vp, _u32, _int, _vp, _int, _int, _vp, _vp, _vp, _vp]),
+    ("fecsim_sim_create", _int, [_vp, _vp, _vp, C.POINTER(_vp)]),
+    ("fecsim_sim_run", _int, [_vp, _u32, _vp]),
+    ("fecsim_sim_stats_get", _int, [_vp, _vp]),
+    ("fecsim_sim_reset", _int, [_vp, _vp]),
+    ("fecsim_sim_free", None, [_vp]),
 ]
 
 

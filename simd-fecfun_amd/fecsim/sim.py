@@ -1,0 +1,200 @@
+"""Host-side mirror of the reference's encoder / channel / error-count interface and
+of its simulation driver loop, over the C ABI.
+
+  Henc.load(path)         <- encode_matrix_allocate()  (src/LDPCEnc/encode_matrix_allocate.cpp:38-92)
+  Henc.gen_codewords(...) <- gen_rand_databits() + LDPCencode()
+                             (src/Top/gen_rand_databits.cpp:11-40, src/LDPCEnc/LDPCencode.cpp:40-100)
+  Henc.encode(...)        <- LDPCencode()
+  Henc.gen_llr(...)       <- normdist2() + gen_metrics()
+                             (src/RNGfunc/prng_functions.cpp:148-186, src/Top/gen_metrics.cpp:18-121)
+  Henc.count_errors(...)  <- LDPC_count_errors()       (src/Top/LDPC_count_errors.cpp:40-136)
+  Sim(...).run(loops)     <- the inner/SNR loops of main() (src/Top/LDPCtop.cpp:86-121)
+
+Codeword bits are int32 tensors [num_cw, mcol, col_words] (row r of a column at word
+r // 32, bit r % 32); col_words = 16 is the reference's cw_bits layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _L
+from .codes import EncTables
+from .decoder import Decoder, _ptr, _row_elems
+
+
+def _torch_dtype(elem):
+    import torch
+    return {_L.I8: torch.int8, _L.I16: torch.int16, _L.I32: torch.int32, _L.F32: torch.float32,
+            _L.F64: torch.float64}[elem]
+
+
+class HencInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("z", "mrow", "mcol", "num_par", "punc_cnt", "fill_cnt", "punc_bits",
+                                         "fill_bits", "col_words", "num_src")]
+
+
+class ChanParams(C.Structure):
+    _fields_ = [("snr_db", C.c_float), ("llr_scale", C.c_float), ("max_abs_llr", C.c_float)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("num_cw", C.c_uint32), ("num_snr", C.c_uint32), ("snr_db", C.POINTER(C.c_float)),
+                ("llr_scale", C.c_float), ("max_abs_llr", C.c_float), ("seed", C.c_uint64), ("use_graph", C.c_int)]
+
+
+class SimStats(C.Structure):
+    _fields_ = [(n, C.c_ulonglong) for n in ("cw", "uncoded_bit_err", "iters", "bit_err", "cw_err")]
+
+
+def chan_sigma(snr_db: float, llr_scale: float = 2.0):
+    """(sigma, LLR factor) of gen_metrics.cpp:28-30 as the library computes them."""
+    p = ChanParams(snr_db, llr_scale, 0.0)
+    s, f = C.c_float(), C.c_float()
+    _L.check(_L.lib().fecsim_chan_sigma(C.byref(p), C.byref(s), C.byref(f)), "fecsim_chan_sigma")
+    return s.value, f.value
+
+
+def _stream(stream):
+    import torch
+    return C.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+
+
+class Henc:
+    """Encoder tables (LDPCencode.raw.dat) held by the library."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = HencInfo()
+        _L.check(_L.lib().fecsim_henc_info(self._h, C.byref(info)), "fecsim_henc_info")
+        self.info = {n: getattr(info, n) for n, _ in HencInfo._fields_}
+
+    @classmethod
+    def load(cls, path: str) -> "Henc":
+        h = C.c_void_p()
+        _L.check(_L.lib().fecsim_henc_load(path.encode(), C.byref(h)), "fecsim_henc_load")
+        return cls(h)
+
+    @classmethod
+    def from_bytes(cls, b: bytes) -> "Henc":
+        words = np.frombuffer(b, dtype="<i4").copy()
+        h = C.c_void_p()
+        _L.check(_L.lib().fecsim_henc_from_words(words.ctypes.data, words.size, C.byref(h)), "fecsim_henc_from_words")
+        return cls(h)
+
+    @classmethod
+    def from_tables(cls, enc: EncTables) -> "Henc":
+        return cls.from_bytes(enc.to_bytes())
+
+    @property
+    def z(self):
+        return self.info["z"]
+
+    @property
+    def mcol(self):
+        return self.info["mcol"]
+
+    @property
+    def n(self):
+        return self.info["z"] * self.info["mcol"]
+
+    @property
+    def col_words(self):
+        return self.info["col_words"]
+
+    def masks(self):
+        """(puncture, filler) masks as uint32 arrays [mcol, col_words]."""
+        shape = (self.info["mcol"], self.info["col_words"])
+        p = np.zeros(shape, np.uint32)
+        f = np.zeros(shape, np.uint32)
+        _L.check(_L.lib().fecsim_henc_masks(self._h, p.ctypes.data, f.ctypes.data), "fecsim_henc_masks")
+        return p, f
+
+    def new_bits(self, num_cw, col_words=None):
+        import torch
+        cw = col_words or self.col_words
+        return torch.zeros((num_cw, self.mcol, cw), dtype=torch.int32, device="cuda")
+
+    def _cw(self, bits, col_words):
+        if bits.dtype.itemsize != 4 or not bits.is_cuda or not bits.is_contiguous():
+            raise TypeError("bits must be a contiguous 32-bit CUDA tensor")
+        cw = col_words or bits.shape[-1]
+        num_cw = bits.numel() // (self.mcol * cw)
+        return num_cw, cw
+
+    def gen_codewords(self, bits, seed: int, cw0: int = 0, col_words=None, stream=None):
+        num_cw, cw = self._cw(bits, col_words)
+        _L.check(_L.lib().fecsim_gen_codewords(self._h, seed, cw0, num_cw, cw, _ptr(bits), _stream(stream)),
+                 "fecsim_gen_codewords")
+
+    def encode(self, bits, col_words=None, stream=None):
+        num_cw, cw = self._cw(bits, col_words)
+        _L.check(_L.lib().fecsim_encode(self._h, num_cw, cw, _ptr(bits), _stream(stream)), "fecsim_encode")
+
+    def gen_llr(self, bits, llr, elem: int, snr_db: float, seed: int, cw0: int = 0, llr_scale: float = 2.0,
+                max_abs_llr: float = 24.0, layout_p: int = 0, uncoded=None, col_words=None, stream=None):
+        num_cw, cw = self._cw(bits, col_words)
+        if llr.dtype != _torch_dtype(elem) or not llr.is_cuda or not llr.is_contiguous():
+            raise TypeError("llr must be a contiguous CUDA tensor of the element kind")
+        need = num_cw * _row_elems(_Info(self), layout_p)
+        if llr.numel() < need:
+            raise ValueError("llr holds fewer than num_cw codewords of this layout")
+        p = ChanParams(snr_db, llr_scale, max_abs_llr)
+        _L.check(_L.lib().fecsim_gen_llr(self._h, C.byref(p), seed, cw0, num_cw, cw, _ptr(bits), elem, layout_p,
+                                         _ptr(llr), _ptr(uncoded), _stream(stream)), "fecsim_gen_llr")
+
+    def count_errors(self, bits, post, elem: int, counts, layout_p: int = 0, cw_bit_err=None, col_words=None,
+                     stream=None):
+        num_cw, cw = self._cw(bits, col_words)
+        if post.dtype != _torch_dtype(elem) or not post.is_cuda or not post.is_contiguous():
+            raise TypeError("post must be a contiguous CUDA tensor of the element kind")
+        if post.numel() < num_cw * _row_elems(_Info(self), layout_p):
+            raise ValueError("post holds fewer than num_cw codewords of this layout")
+        _L.check(_L.lib().fecsim_count_errors(self._h, num_cw, cw, _ptr(bits), elem, layout_p, _ptr(post),
+                                              _ptr(counts), _ptr(cw_bit_err), _stream(stream)),
+                 "fecsim_count_errors")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _L._lib is not None:
+            _L.lib().fecsim_henc_free(self._h)
+            self._h = None
+
+
+class _Info:
+    """Adapter so decoder._row_elems can size buffers of an encoder's code."""
+
+    def __init__(self, henc):
+        self.info = {"z": henc.z, "mcol": henc.mcol}
+
+
+class Sim:
+    """The simulation driver loop on the device: per inner loop, new codewords, then
+    for every SNR point LLRs (same noise), decode, error count."""
+
+    def __init__(self, henc: Henc, dec: Decoder, num_cw: int, snr_db, llr_scale: float = 2.0,
+                 max_abs_llr: float = 24.0, seed: int = 100, use_graph: bool = True):
+        self.henc, self.dec = henc, dec          # the handles must outlive the simulation
+        self.snr = np.asarray(snr_db, np.float32)
+        p = SimParams(num_cw, len(self.snr), self.snr.ctypes.data_as(C.POINTER(C.c_float)), llr_scale, max_abs_llr,
+                      seed, int(use_graph))
+        s = C.c_void_p()
+        _L.check(_L.lib().fecsim_sim_create(henc._h, dec._d, C.byref(p), C.byref(s)), "fecsim_sim_create")
+        self._s = s
+
+    def run(self, loops: int = 1, stream=None):
+        _L.check(_L.lib().fecsim_sim_run(self._s, loops, _stream(stream)), "fecsim_sim_run")
+
+    def reset(self, stream=None):
+        _L.check(_L.lib().fecsim_sim_reset(self._s, _stream(stream)), "fecsim_sim_reset")
+
+    def stats(self):
+        """list of dicts per SNR point: cw, uncoded_bit_err, iters, bit_err, cw_err"""
+        arr = (SimStats * len(self.snr))()
+        _L.check(_L.lib().fecsim_sim_stats_get(self._s, arr), "fecsim_sim_stats_get")
+        return [{n: getattr(x, n) for n, _ in SimStats._fields_} for x in arr]
+
+    def __del__(self):
+        if getattr(self, "_s", None) and _L._lib is not None:
+            _L.lib().fecsim_sim_free(self._s)
+            self._s = None

@@ -1,0 +1,243 @@
+"""GPU parity of the simulation kernels (through the C ABI) against the CPU oracle:
+data bits + encoder, noise + quantized LLRs, error counting, and the device
+simulation loop (eager and HIP-graph replays) -- all bit-exact."""
+import sys
+
+import numpy as np
+import pytest
+
+from fecsim import codes
+from fecsim.decoder import Decoder, Hdec
+from fecsim.sim import Henc, Sim, chan_sigma
+import refsim
+from refsim import O
+
+pytestmark = pytest.mark.gpu
+
+# (matrix set, num_par, z_over): Z < 32, Z multiple of 32, odd word counts, lifted
+# num_par, 11ay lifting, 5G NR with puncturing and filler bits, BG1 Z=384
+ENC_CASES = [
+    ("11ad 1/2", 1, None),
+    ("11ad 13/16", 1, None),
+    ("11n 1/2 z27", 1, None),
+    ("11n 5/6 z81", 1, None),
+    ("16e 2/3A", 1, 96),
+    ("16e 1/2", 1, 64),
+    ("ITU G.hn 1/2 z14", 1, None),
+    ("11ay 7/8", 1, None),
+    ("11ad 1/2", 3, None),
+    ("5GNR_n600_k300", 1, None),
+    ("5GNR_n500_k220", 1, None),
+    ("5GNR_n25344_k8448", 1, None),
+]
+
+
+def _say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _cs(name, npar=1, zo=None):
+    return codes.gen_structs(name, num_par=npar, simd_size=512, elem_size=8, z_over=zo)
+
+
+def _oracle_codewords(cs, seed, cw0, ncw):
+    """[ncw][mcol][64] byte words: the oracle's counter-based data bits + encoder."""
+    z, mrow, mcol = cs.z, cs.enc.mrow, cs.enc.mcol
+    _, fill = refsim.word_masks(cs.enc)
+    bits = np.zeros((ncw, mcol, 64), np.uint8)
+    O.lib().orc_gpu_databits(seed, cw0, ncw, z, mrow, mcol, fill.ctypes.data, bits.ctypes.data)
+    cnt, colnum, sc, ss = refsim.enc_arrays(cs.enc)
+    O.lib().orc_encode(ncw, z, mrow, mcol, cnt.ctypes.data, colnum.ctypes.data, sc.ctypes.data, ss.ctypes.data,
+                       bits.ctypes.data)
+    return bits
+
+
+def _words(t, mcol):
+    """GPU bits [ncw, mcol, col_words] int32 -> [ncw][mcol][64] byte words."""
+    a = t.cpu().numpy().view(np.uint32)
+    out = np.zeros((a.shape[0], mcol, 16), np.uint32)
+    k = min(16, a.shape[2])
+    out[:, :, :k] = a[:, :, :k]
+    return out.view(np.uint8)
+
+
+def _dense_masks(cs):
+    pm, fm = cs.enc.masks()
+    return pm.reshape(-1).astype(np.uint8), fm.reshape(-1).astype(np.uint8)
+
+
+def _syndrome_zero(cs, dense):
+    """H c = 0 over GF(2) from the base matrix (block (i, j) with shift s:
+    check row i*Z + r sees bit j*Z + (r + s) mod Z)."""
+    z = cs.z
+    hc = cs.hc
+    b = dense.reshape(dense.shape[0], hc.shape[1], z)
+    r = np.arange(z)
+    for i in range(hc.shape[0]):
+        acc = np.zeros((dense.shape[0], z), np.uint8)
+        for j in range(hc.shape[1]):
+            if hc[i, j] >= 0:
+                acc ^= b[:, j, (r + hc[i, j]) % z]
+        if acc.any():
+            return False
+    return True
+
+
+@pytest.mark.parametrize("name,npar,zo", ENC_CASES)
+def test_gpu_gen_codewords_matches_oracle(name, npar, zo, cuda):
+    import torch
+    cs = _cs(name, npar, zo)
+    h = Henc.from_tables(cs.enc)
+    ncw = 37 if cs.z < 300 else 5
+    _say("gen_codewords", name, npar, zo, "z", cs.z, "ncw", ncw)
+    want = _oracle_codewords(cs, 1234, 555, ncw)
+    for cwords in (h.col_words, 16):
+        bits = torch.full((ncw, cs.enc.mcol, cwords), -1, dtype=torch.int32, device="cuda")
+        h.gen_codewords(bits, seed=1234, cw0=555)
+        torch.cuda.synchronize()
+        got = _words(bits, cs.enc.mcol)
+        assert np.array_equal(got, want), (name, cwords)
+        if cwords > h.col_words:   # words past ceil(z/32) are cleared
+            assert not bits[:, :, h.col_words:].any().item()
+    dense = refsim.unpack_bits(want, cs.z, cs.enc.mcol)
+    assert _syndrome_zero(cs, dense)
+
+
+@pytest.mark.parametrize("name,npar,zo", [c for c in ENC_CASES if c[0] != "5GNR_n25344_k8448"])
+def test_gpu_encode_reference_data_stream(name, npar, zo, cuda):
+    """fecsim_encode on the reference's own data bits (xoshiro256++ stream of
+    gen_rand_databits) in the reference's 64-byte word layout."""
+    import torch
+    cs = _cs(name, npar, zo)
+    h = Henc.from_tables(cs.enc)
+    rs = refsim.RefStream(cs)
+    _, want = rs.inner_loop(23)
+    data = want.copy()
+    data[:, cs.enc.mcol - cs.enc.mrow:, :] = 0
+    bits = torch.from_numpy(data.view(np.int32).reshape(23, cs.enc.mcol, 16).copy()).cuda()
+    h.encode(bits)
+    torch.cuda.synchronize()
+    assert np.array_equal(_words(bits, cs.enc.mcol), want)
+
+
+@pytest.mark.parametrize("name,npar,zo", [("11ad 1/2", 1, None), ("5GNR_n600_k300", 1, None),
+                                         ("11n 1/2 z27", 2, None), ("16e 1/2", 1, 92)])
+@pytest.mark.parametrize("elem", [O.I8, O.F32, O.I16])
+def test_gpu_llr_matches_oracle(name, npar, zo, elem, cuda):
+    import torch
+    cs = _cs(name, npar, zo)
+    h = Henc.from_tables(cs.enc)
+    ncw, seed, cw0, snr = 29, 99, 7000, -0.75
+    scale, mx = (1.0, 1e9) if elem == O.F32 else (2.0, 24.0)
+    _say("llr", name, npar, zo, "elem", elem)
+    bits = h.new_bits(ncw)
+    h.gen_codewords(bits, seed=seed, cw0=cw0)
+    dense = refsim.unpack_bits(_words(bits, cs.enc.mcol), cs.z, cs.enc.mcol)
+    punct, fill = _dense_masks(cs)
+    sigma, f = chan_sigma(snr, scale)
+    want, werr = O.noise_llr(seed, cw0, dense, cs.z, cs.enc.mcol, sigma, f, mx, punct, fill, elem)
+    tdt = {O.I8: torch.int8, O.I16: torch.int16, O.F32: torch.float32}[elem]
+    llr = torch.zeros((ncw, cs.enc.mcol * cs.z), dtype=tdt, device="cuda")
+    unc = torch.zeros(1, dtype=torch.int64, device="cuda")
+    h.gen_llr(bits, llr, elem, snr, seed=seed, cw0=cw0, llr_scale=scale, max_abs_llr=mx, uncoded=unc)
+    torch.cuda.synchronize()
+    assert np.array_equal(llr.cpu().numpy().view(np.uint8), want.view(np.uint8))
+    assert int(unc.item()) == werr
+    # the reference layout (P = 64 lanes): same values at +P + c*(ceil(z/P)+2)*P
+    P = 64
+    spz = (cs.z + P - 1) // P
+    ref = torch.zeros((ncw, cs.enc.mcol * (spz + 2) * P), dtype=tdt, device="cuda")
+    h.gen_llr(bits, ref, elem, snr, seed=seed, cw0=cw0, llr_scale=scale, max_abs_llr=mx, layout_p=P)
+    torch.cuda.synchronize()
+    assert np.array_equal(O.ref_to_dense(ref.cpu().numpy(), cs.z, cs.enc.mcol, P).view(np.uint8),
+                          want.view(np.uint8))
+
+
+@pytest.mark.parametrize("name,npar", [("11ad 1/2", 1), ("11ad 1/2", 3), ("5GNR_n600_k300", 1)])
+def test_gpu_count_errors_matches_oracle(name, npar, cuda):
+    import torch
+    cs = _cs(name, npar)
+    h = Henc.from_tables(cs.enc)
+    ncw, seed = 301, 5
+    bits = h.new_bits(ncw)
+    h.gen_codewords(bits, seed=seed, cw0=0)
+    words = _words(bits, cs.enc.mcol)
+    dense = refsim.unpack_bits(words, cs.z, cs.enc.mcol)
+    for snr in (-3.0, 1.0):
+        llr = torch.zeros((ncw, cs.enc.mcol * cs.z), dtype=torch.int8, device="cuda")
+        h.gen_llr(bits, llr, O.I8, snr, seed=seed)
+        counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+        per = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        h.count_errors(bits, llr, O.I8, counts, cw_bit_err=per)
+        torch.cuda.synchronize()
+        post = llr.cpu().numpy()
+        ce, be = refsim.count_errors(cs, words, post)
+        assert (int(counts[1].item()), int(counts[0].item())) == (ce, be)
+        k = (cs.enc.mcol - cs.enc.mrow) * cs.z
+        assert np.array_equal(per.cpu().numpy(), ((post[:, :k] < 0) != (dense[:, :k] == 1)).sum(1))
+        _say("count", name, npar, snr, "bit errors", be, "cw errors", ce)
+    # float posteriors in the reference layout; -0.0 is a 0 decision
+    P = 16
+    spz = (cs.z + P - 1) // P
+    postf = np.where(dense == 1, np.float32(-1.5), np.float32(0.0)).astype(np.float32)
+    postf[3, 5] = np.float32(2.0) if dense[3, 5] == 1 else np.float32(-2.0)   # one error
+    if dense[4, 6] == 0:
+        postf[4, 6] = np.float32(-0.0)                                       # no error
+    refl = np.zeros((ncw, cs.enc.mcol, (spz + 2) * P), np.float32)
+    refl[:, :, P:P + cs.z] = postf.reshape(ncw, cs.enc.mcol, cs.z)
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    h.count_errors(bits, torch.from_numpy(refl.reshape(ncw, -1)).cuda(), O.F32, counts, layout_p=P)
+    torch.cuda.synchronize()
+    ce, be = refsim.count_errors(cs, words, postf, elem=O.F32)
+    assert (int(counts[1].item()), int(counts[0].item())) == (ce, be) == (1, 1)
+
+
+def _oracle_sim(cs, kind, snrs, seed, num_cw, loops, max_iter=10, mrm=31.0, beta=1.0, scale=2.0, mx=24.0):
+    elem = O.ELEM_OF_CHAR[kind[kind.index("Vec") + 3:].lstrip("0123456789")[0]]
+    punct, fill = _dense_masks(cs)
+    st = [dict(cw=0, uncoded_bit_err=0, iters=0, bit_err=0, cw_err=0) for _ in snrs]
+    for l in range(loops):
+        words = _oracle_codewords(cs, seed, l * num_cw, num_cw)
+        dense = refsim.unpack_bits(words, cs.z, cs.enc.mcol)
+        for i, snr in enumerate(snrs):
+            sigma, f = chan_sigma(snr, scale)
+            llr, err = O.noise_llr(seed, l * num_cw, dense, cs.z, cs.enc.mcol, sigma, f, mx, punct, fill, elem)
+            post, _, _, tot = O.decode_ideal(cs.dec, llr, elem=elem, max_iter=max_iter, max_row_metric=mrm,
+                                             beta=beta, want_c2v=False)
+            ce, be = refsim.count_errors(cs, words, post, elem=elem)
+            s = st[i]
+            s["cw"] += num_cw
+            s["uncoded_bit_err"] += err
+            s["iters"] += tot
+            s["bit_err"] += be
+            s["cw_err"] += ce
+    return st
+
+
+@pytest.mark.parametrize("name,kind,graph", [("11ad 1/2", "Vec64c_min", False), ("11ad 1/2", "Vec64c_min", True),
+                                             ("11n 1/2 z27", "Vec64c_min", True),
+                                             ("5GNR_n600_k300", "Vec32s_min", True),
+                                             ("16e 1/2", "Vec16f_min", False)])
+def test_gpu_sim_loop_matches_oracle_chain(name, kind, graph, cuda):
+    import torch
+    cs = _cs(name)
+    henc = Henc.from_tables(cs.enc)
+    dec = Decoder(Hdec.from_tables(cs.dec), kind, 10, 31.0, 1.0)
+    snrs = [-1.0, 0.0, 1.5]
+    num_cw, loops, seed = 400, 3, 31
+    _say("sim", name, kind, "graph", graph)
+    sim = Sim(henc, dec, num_cw, snrs, seed=seed, use_graph=graph)
+    sim.run(1)
+    torch.cuda.synchronize()
+    torch.zeros(1000, device="cuda").sum()   # eager work between graph replays
+    sim.run(loops - 1)
+    got = sim.stats()
+    want = _oracle_sim(cs, kind, snrs, seed, num_cw, loops)
+    assert got == want, (got, want)
+    # reset clears the counters; the codeword index keeps advancing (loop 3 next)
+    sim.reset()
+    sim.run(1)
+    got = sim.stats()
+    full = _oracle_sim(cs, kind, snrs, seed, num_cw, loops + 1)
+    for g, w4, w3 in zip(got, full, want):
+        assert g == {k: w4[k] - w3[k] for k in g}
