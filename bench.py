@@ -4,9 +4,12 @@
 Workload (the reference's default run, build/run_LDPCsim.pl:8-39 / README.md:41-47):
 beta 1, row cap 31, at most 10 iterations with syndrome early exit, channel LLRs
 quantized to int8 (scale 2, |LLR| <= 24) for SNR -1.00 .. 0.00 dB in 0.25 dB steps,
-1e6 codewords per SNR point.  One step = decoding all five SNR batches (5e6 cw),
-decode only, inputs resident in HBM (the reference times LDPCdecode_set alone,
-src/Top/LDPCtop.cpp:107-110).
+1e6 codewords per SNR point, random codewords (seed 100) from the library's encoder
+and noise generator.  One step = decoding all five SNR batches (5e6 cw), decode
+only, inputs resident in HBM (the reference times LDPCdecode_set alone,
+src/Top/LDPCtop.cpp:107-110).  The end-to-end simulation loop (new codewords,
+LLRs, decode, error count per SNR point; LDPCtop.cpp:86-121) is timed after it
+and reported as sim_cw_per_s.
 
 Prints one JSON line.  The CPU baseline (the oracle's SIMD restatement of
 LDPCtop_Vec64c_min on the host cores) is timed in the same run on a bounded sample.
@@ -27,23 +30,30 @@ for _p in (os.path.join(REPO, "simd-fecfun_amd"), os.path.join(REPO, "oracle"), 
 METRIC = "decoded info Gbit/s + CW/s, int8 min-sum 11ad 1/2; dB gain of learned offsets"
 SNRS = (-1.0, -0.75, -0.5, -0.25, 0.0)
 K_INFO = 336
+SEED = 100                                  # run_LDPCsim.pl:36
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz, 32-bit lane-ops
 OPS_PER_EDGE_UPDATE = 13                    # algorithmic int ops per edge-update (offset min-sum)
 E_11AD = 2184                               # edges per iteration, 11ad 1/2
+# HBM bytes per codeword moved by one k_pair_i8 launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs scaled by a calibration copy (profiles/k_pair_i8_11ad12_1e6cw_hbm_traffic.md)
+PAIR_TRAFFIC_BYTES_PER_CW = 1354.1
 
 
-def make_llrs(cs, ncw, snr, seed, torch, llr_scale=2.0, max_abs=24.0):
-    """Random codewords, BPSK over AWGN, quantized like gen_metrics.cpp:76-98, built on
-    the GPU with the library's encoder/noise path when present, else an all-zero
-    codeword with torch noise (equivalent error statistics for a linear code)."""
-    from refsim import O
-    sigma, f = O.snr_params(snr, llr_scale)
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    n = torch.randn(ncw, cs.n, device="cuda", generator=g)
-    y = torch.addcmul(torch.ones_like(n), n, torch.tensor(sigma, device="cuda"))
-    llr = torch.clamp(y * f, -max_abs, max_abs)
-    return torch.round(llr).to(torch.int8).contiguous()
+def make_inputs(henc, ncw, seed, torch, llr_scale=2.0, max_abs=24.0):
+    """Random codewords (counter-based data bits + QC encoder), BPSK over AWGN,
+    quantized like gen_metrics.cpp:76-98, all generated on the GPU by the library.
+    As the reference does (LDPCtop.cpp:90-97), the codewords and the noise are the
+    same at every SNR point."""
+    from fecsim import lib as _L
+    bits = henc.new_bits(ncw)
+    henc.gen_codewords(bits, seed=seed, cw0=0)
+    llrs = []
+    for snr in SNRS:
+        x = torch.empty((ncw, henc.n), dtype=torch.int8, device="cuda")
+        henc.gen_llr(bits, x, _L.I8, snr, seed=seed, cw0=0, llr_scale=llr_scale, max_abs_llr=max_abs)
+        llrs.append(x)
+    return bits, llrs
 
 
 def cpu_baseline(cs, llr_samples, max_seconds=12.0):
@@ -79,6 +89,44 @@ def cpu_baseline(cs, llr_samples, max_seconds=12.0):
     }
 
 
+def probe(args):
+    import torch
+    from fecsim import codes, lib as _L
+    from fecsim.decoder import Decoder, Hdec
+    from fecsim.sim import Henc, Sim
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    henc = Henc.from_tables(cs.enc)
+    dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+    ncw = args.ncw
+    if args.probe == "traffic":
+        bits = henc.new_bits(ncw)
+        henc.gen_codewords(bits, seed=SEED)
+        x = torch.empty((ncw, cs.n), dtype=torch.int8, device="cuda")
+        henc.gen_llr(bits, x, _L.I8, -0.5, seed=SEED)
+        o = torch.empty_like(x)
+        it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(2):
+            dec.decode(x, o, iters=it)
+            torch.cuda.synchronize()
+        for _ in range(2):
+            o.copy_(x)
+            torch.cuda.synchronize()
+        print(json.dumps({"probe": "traffic", "ncw": ncw, "copy_bytes": x.numel(),
+                          "mean_iterations": float(it.float().mean())}))
+        return
+    sim = Sim(henc, dec, ncw, list(SNRS), seed=SEED, use_graph=False)
+    sim.run(1)
+    torch.cuda.synchronize()
+    sim.reset()
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    st = sim.stats()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"probe": "sim", "ncw": ncw, "loops": args.steps,
+                      "sim_cw_per_s": ncw * len(SNRS) * args.steps / dt, "stats": st}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,10 +137,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel", choices=["auto", "generic", "pair"], default="auto",
                     help="decoder kernel (results are identical)")
+    ap.add_argument("--probe", choices=["traffic", "sim"], default=None,
+                    help="short fixed workloads for profiler runs instead of the benchmark: 'traffic' = "
+                         "one decode of --ncw codewords at -0.5 dB twice, then a plain copy of the LLR "
+                         "tensor twice (counter calibration); 'sim' = --steps simulation loops")
     args = ap.parse_args()
+    if args.probe:
+        return probe(args)
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(codeword batches shard trivially; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (codewords are "
+              "independent, so a batch would shard by codeword range with a final sum of "
+              "the error counters)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -107,7 +162,9 @@ def main():
     kname = "k_pair_i8<PairShape0> (row-pair, persistent)" if dec.kernel == _L.KERNEL_PAIR else "k_decode<IntPolicy<8>,8>"
     ncw = args.ncw
     print("bench: generating %d x %d codeword LLRs on the GPU" % (len(SNRS), ncw), file=sys.stderr, flush=True)
-    llrs = [make_llrs(cs, ncw, s, 100 + i, torch) for i, s in enumerate(SNRS)]
+    from fecsim.sim import Henc, Sim
+    henc = Henc.from_tables(cs.enc)
+    bits, llrs = make_inputs(henc, ncw, SEED, torch)
     outs = [torch.empty_like(x) for x in llrs]
     iters = [torch.zeros(ncw, dtype=torch.int32, device="cuda") for _ in SNRS]
     torch.cuda.synchronize()
@@ -148,6 +205,30 @@ def main():
     valu_tops = edge_updates_per_launch * OPS_PER_EDGE_UPDATE / (avg_launch_ms * 1e-3) / 1e12
     mean_iters = {"%g" % s: float(it.float().mean().item()) for s, it in zip(SNRS, iters)}
 
+    # error rates of the decoded batch (data columns vs transmitted bits)
+    from fecsim import lib as _L
+    rates = {}
+    for s, o in zip(SNRS, outs):
+        cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+        henc.count_errors(bits, o, _L.I8, cnt)
+        be, ce = (int(v) for v in cnt.tolist())
+        rates["%g" % s] = {"ber": be / (ncw * K_INFO), "cwer": ce / ncw}
+
+    # end-to-end simulation loop: new codewords + LLRs + decode + count for all SNR
+    # points, one graph replay per loop
+    print("bench: simulation loop", file=sys.stderr, flush=True)
+    del outs
+    sim = Sim(henc, dec, ncw, list(SNRS), seed=SEED, use_graph=True)
+    sim.run(1)
+    torch.cuda.synchronize()
+    sim.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    torch.cuda.synchronize()
+    sim_wall = time.perf_counter() - t0
+    sim_cw_s = ncw * len(SNRS) * args.steps / sim_wall
+
     res = {
         "metric": METRIC, "value": gbit, "unit": "info Gbit/s", "cw_per_s": cw_s,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
@@ -156,10 +237,16 @@ def main():
         "dtype": "int8", "data": "synthetic",
         "config": {"workload": "11ad 1/2 (N=672,K=336,Z=42) int8 offset min-sum decode, beta 1, cap 31, max 10 "
                                "iterations with syndrome exit, SNR -1.00..0.00 dB step 0.25, %d cw per SNR" % ncw,
-                   "codewords_per_step": cw_per_step, "input": "all-zero codeword + AWGN, quantized LLRs"},
+                   "codewords_per_step": cw_per_step,
+                   "input": "random codewords (seed %d, library encoder) + AWGN, int8 LLRs (scale 2, max 24)" % SEED},
         "mean_iterations": mean_iters,
+        "error_rates": rates,
+        "sim_cw_per_s": sim_cw_s,
+        "sim_info_gbit_s": sim_cw_s * K_INFO / 1e9,
+        "sim_note": "encode + WGN/LLR + decode + count per codeword and SNR point, HIP graph per loop",
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": hbm_gbs / HBM_PEAK_GBS,
+                     "traffic": PAIR_TRAFFIC_BYTES_PER_CW * ncw if dec.kernel == _L.KERNEL_PAIR else None,
                      "kernel": kname, "avg_launch_ms": avg_launch_ms,
                      "algorithmic_bytes_per_launch": hbm_bytes},
         "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",

@@ -210,6 +210,21 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
     fecsim_decoder_free(dec);
     return hip_fail(he, "decoder tables to device");
   }
+  if (d.shape.c2v_global) {
+    // one C2V slot per resident (block, wave, codeword slot); the kernel strides over the batch
+    const int blocks = generic_resident_blocks(d);
+    if (blocks < 1) {
+      fecsim_decoder_free(dec);
+      return fail(FECSIM_E_HIP, "occupancy query failed");
+    }
+    d.max_grid = (uint32_t)blocks;
+    const size_t bytes = (size_t)blocks * d.shape.waves * d.shape.cw_per_wave * d.shape.c2v_stride * d.elem_bytes;
+    he = hipMalloc(&d.d_c2v_scratch, bytes);
+    if (he != hipSuccess) {
+      fecsim_decoder_free(dec);
+      return hip_fail(he, "C2V scratch");
+    }
+  }
   d.pair_shape = pair_i8_shape(d);
   if (d.pair_shape >= 0) {
     pair_layout(h, &d.pair);
@@ -254,6 +269,7 @@ void fecsim_decoder_free(fecsim_decoder* d) {
   if (d->d.d_edges) (void)hipFree(d->d.d_edges);
   if (d->d.d_layer_w) (void)hipFree(d->d.d_layer_w);
   if (d->d.d_pair_adr) (void)hipFree(d->d.d_pair_adr);
+  if (d->d.d_c2v_scratch) (void)hipFree(d->d.d_c2v_scratch);
   delete d;
 }
 

@@ -5,7 +5,10 @@
 
 namespace fecsim {
 
-template <class P, int MAXW>
+// GC2V: the C2V messages live in a global scratch slot per (block, wave, codeword
+// slot) instead of the wave's LDS slice; the grid then walks the codewords with a
+// stride so the scratch is sized by the grid, not by the batch.
+template <class P, int MAXW, bool GC2V>
 __global__ void __launch_bounds__(256) k_decode(const KernelArgs a) {
   using S = typename P::S;
   HIP_DYNAMIC_SHARED(char, smem);
@@ -14,42 +17,50 @@ __global__ void __launch_bounds__(256) k_decode(const KernelArgs a) {
   const int L = d.lanes_per_cw, CPW = d.cw_per_wave, Z = d.z;
   const int slot = lane / L;
   const int lr = lane - slot * L;
-  const long long cw = ((long long)blockIdx.x * d.waves + wave) * CPW + slot;
-  const bool valid = slot < CPW && cw < (long long)a.num_cw;
   const int sl = slot < CPW ? slot : 0;
   char* wbase = smem + (size_t)wave * d.wave_bytes;
   S* post = reinterpret_cast<S*>(wbase) + (size_t)sl * d.post_stride;
-  S* c2v = reinterpret_cast<S*>(wbase) + (size_t)CPW * d.post_stride + (size_t)sl * d.c2v_stride;
+  S* c2v;
+  if constexpr (GC2V)
+    c2v = reinterpret_cast<S*>(d.c2v_scratch) +
+          (((size_t)blockIdx.x * d.waves + wave) * CPW + sl) * (size_t)d.c2v_stride;
+  else
+    c2v = reinterpret_cast<S*>(wbase) + (size_t)CPW * d.post_stride + (size_t)sl * d.c2v_stride;
   const S* gin = reinterpret_cast<const S*>(a.metrics_in);
   S* gout = reinterpret_cast<S*>(a.metrics_out);
+  unsigned long long tot = 0;
 
-  if (valid) {
-    const long long cbase = a.base + cw * a.cw_stride;
-    for (int c = 0; c < d.mcol; ++c)
-      for (int r = lr; r < Z; r += L) post[c * Z + r] = gin[cbase + c * a.col_stride + r];
-  }
-  const uint32_t iter = decode_loop<P, MAXW>(d, post, c2v, slot, lr, valid);
+  for (long long first = ((long long)blockIdx.x * d.waves + wave) * CPW; first < (long long)a.num_cw;
+       first += (long long)gridDim.x * d.waves * CPW) {   // uniform per wave
+    const long long cw = first + slot;
+    const bool valid = slot < CPW && cw < (long long)a.num_cw;
+    if (valid) {
+      const long long cbase = a.base + cw * a.cw_stride;
+      for (int c = 0; c < d.mcol; ++c)
+        for (int r = lr; r < Z; r += L) post[c * Z + r] = gin[cbase + c * a.col_stride + r];
+    }
+    const uint32_t iter = decode_loop<P, MAXW>(d, post, c2v, slot, lr, valid);
 
-  if (valid) {
-    const long long cbase = a.base + cw * a.cw_stride;
-    for (int c = 0; c < d.mcol; ++c)
-      for (int r = lr; r < Z; r += L) gout[cbase + c * a.col_stride + r] = post[c * Z + r];
-    if (a.c2v) {
-      S* gc = reinterpret_cast<S*>(a.c2v) + cw * (long long)d.e;
-      for (int i = lr; i < d.e; i += L) gc[i] = c2v[i];
+    if (valid) {
+      const long long cbase = a.base + cw * a.cw_stride;
+      for (int c = 0; c < d.mcol; ++c)
+        for (int r = lr; r < Z; r += L) gout[cbase + c * a.col_stride + r] = post[c * Z + r];
+      if (a.c2v) {
+        S* gc = reinterpret_cast<S*>(a.c2v) + cw * (long long)d.e;
+        for (int i = lr; i < d.e; i += L) gc[i] = c2v[i];
+      }
+      if (lr == 0 && a.iters) a.iters[cw] = iter;
     }
-    if (lr == 0 && a.iters) a.iters[cw] = iter;
-  }
-  if (a.tot_iter) {
-    // one atomic per wave: lane 0 sums the slots' iteration counts
-    unsigned long long s = 0;
-    for (int q = 0; q < CPW; ++q) {
-      const unsigned it = (unsigned)__shfl((int)iter, q * L, 64);
-      const long long cq = ((long long)blockIdx.x * d.waves + wave) * CPW + q;
-      if (cq < (long long)a.num_cw) s += it;
+    if (a.tot_iter) {
+      // lane 0 sums the slots' iteration counts
+      for (int q = 0; q < CPW; ++q) {
+        const unsigned it = (unsigned)__shfl((int)iter, q * L, 64);
+        if (first + q < (long long)a.num_cw) tot += it;
+      }
     }
-    if (lane == 0 && s) atomicAdd(a.tot_iter, s);
+    wave_sync();
   }
+  if (a.tot_iter && lane == 0 && tot) atomicAdd(a.tot_iter, tot);   // one atomic per wave
 }
 
 // ---------------------------------------------------------------------------
@@ -78,12 +89,22 @@ int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecSha
   s->post_stride = (N + pad - 1) / pad * pad;
   s->c2v_stride = (E + pad - 1) / pad * pad;
   const int lds_cap = 64 * 1024;
+  const int cpw0 = s->cw_per_wave;
+  s->c2v_global = false;
   for (;;) {
     s->wave_bytes = s->cw_per_wave * ((s->post_stride + s->c2v_stride) * elem_bytes + extra_slot_bytes);
     if (s->wave_bytes <= lds_cap || s->cw_per_wave == 1) break;
     --s->cw_per_wave;
   }
-  if (s->wave_bytes > lds_cap) { *why = "codeword does not fit one wave's LDS slice"; return FECSIM_E_UNSUPPORTED; }
+  if (s->wave_bytes > lds_cap) {
+    // posteriors in LDS, C2V in a global scratch slot
+    s->c2v_global = true;
+    for (s->cw_per_wave = cpw0;; --s->cw_per_wave) {
+      s->wave_bytes = s->cw_per_wave * (s->post_stride * elem_bytes + extra_slot_bytes);
+      if (s->wave_bytes <= lds_cap || s->cw_per_wave == 1) break;
+    }
+    if (s->wave_bytes > lds_cap) { *why = "codeword posteriors do not fit one wave's LDS slice"; return FECSIM_E_UNSUPPORTED; }
+  }
   s->waves = lds_cap / s->wave_bytes;
   if (s->waves > 4) s->waves = 4;
   return FECSIM_OK;
@@ -113,6 +134,7 @@ void fill_core(const DecoderDev& dv, DecCore* c) {
   c->c2v_stride = dv.shape.c2v_stride;
   c->wave_bytes = dv.shape.wave_bytes;
   c->max_iter = dv.params.max_iter;
+  c->c2v_scratch = dv.d_c2v_scratch;
   const float mrm = dv.params.max_row_metric, beta = dv.params.beta_offset;
   c->ibeta = c->icap = 0;
   c->fbeta = c->fcap = 0.0;
@@ -135,14 +157,58 @@ void fill_core(const DecoderDev& dv, DecCore* c) {
   }
 }
 
+template <class P, bool G>
+static const void* kernel_of(int maxw) {
+  switch (maxw) {
+  case 8: return (const void*)k_decode<P, 8, G>;
+  case 16: return (const void*)k_decode<P, 16, G>;
+  case 24: return (const void*)k_decode<P, 24, G>;
+  default: return (const void*)k_decode<P, 32, G>;
+  }
+}
+
+template <class P>
+static const void* kernel_for(const DecoderDev& d) {
+  return d.shape.c2v_global ? kernel_of<P, true>(d.shape.maxw) : kernel_of<P, false>(d.shape.maxw);
+}
+
+static const void* generic_kernel(const DecoderDev& d) {
+  switch (d.elem) {
+  case FECSIM_I8: return kernel_for<IntPolicy<8>>(d);
+  case FECSIM_I16: return kernel_for<IntPolicy<16>>(d);
+  case FECSIM_I32: return kernel_for<IntPolicy<32>>(d);
+  case FECSIM_F32:
+    return d.mode == FECSIM_PROD ? kernel_for<FloatPolicy<float, true>>(d) : kernel_for<FloatPolicy<float, false>>(d);
+  default:
+    return d.mode == FECSIM_PROD ? kernel_for<FloatPolicy<double, true>>(d) : kernel_for<FloatPolicy<double, false>>(d);
+  }
+}
+
+int generic_resident_blocks(const DecoderDev& d) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const size_t lds = (size_t)d.shape.waves * d.shape.wave_bytes;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, generic_kernel(d), 64 * d.shape.waves, lds) !=
+          hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return per_cu * cus;
+}
+
 template <class P>
 static hipError_t launch_p(const DecoderDev& d, const KernelArgs& a, hipStream_t st) {
   const long long per_block = (long long)a.core.waves * a.core.cw_per_wave;
-  const unsigned blocks = (unsigned)((a.num_cw + per_block - 1) / per_block);
+  long long blocks = (a.num_cw + per_block - 1) / per_block;
   if (blocks == 0) return hipSuccess;
-  const dim3 grid(blocks), block(64 * a.core.waves);
+  if (d.max_grid && blocks > (long long)d.max_grid) blocks = d.max_grid;
+  const dim3 grid((unsigned)blocks), block(64 * a.core.waves);
   const size_t lds = (size_t)a.core.waves * a.core.wave_bytes;
-  FECSIM_MAXW_DISPATCH(d.shape.maxw, k_decode, P, grid, block, lds, st, a);
+  if (d.shape.c2v_global) {
+    if (!d.d_c2v_scratch) return hipErrorInvalidValue;
+    FECSIM_MAXW_DISPATCH(d.shape.maxw, k_decode, P, true, grid, block, lds, st, a);
+  } else {
+    FECSIM_MAXW_DISPATCH(d.shape.maxw, k_decode, P, false, grid, block, lds, st, a);
+  }
   return hipGetLastError();
 }
 

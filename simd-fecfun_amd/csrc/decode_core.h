@@ -248,12 +248,12 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
 }
 
 // dispatch on the row-weight bucket of a decoder
-#define FECSIM_MAXW_DISPATCH(maxw, KERNEL, P, grid, block, lds, st, ...)                 \
-  switch (maxw) {                                                                         \
-  case 8: hipLaunchKernelGGL((KERNEL<P, 8>), grid, block, lds, st, __VA_ARGS__); break;   \
-  case 16: hipLaunchKernelGGL((KERNEL<P, 16>), grid, block, lds, st, __VA_ARGS__); break; \
-  case 24: hipLaunchKernelGGL((KERNEL<P, 24>), grid, block, lds, st, __VA_ARGS__); break; \
-  default: hipLaunchKernelGGL((KERNEL<P, 32>), grid, block, lds, st, __VA_ARGS__); break; \
+#define FECSIM_MAXW_DISPATCH(maxw, KERNEL, P, G, grid, block, lds, st, ...)                 \
+  switch (maxw) {                                                                            \
+  case 8: hipLaunchKernelGGL((KERNEL<P, 8, G>), grid, block, lds, st, __VA_ARGS__); break;   \
+  case 16: hipLaunchKernelGGL((KERNEL<P, 16, G>), grid, block, lds, st, __VA_ARGS__); break; \
+  case 24: hipLaunchKernelGGL((KERNEL<P, 24, G>), grid, block, lds, st, __VA_ARGS__); break; \
+  default: hipLaunchKernelGGL((KERNEL<P, 32, G>), grid, block, lds, st, __VA_ARGS__); break; \
   }
 
 }  // namespace fecsim

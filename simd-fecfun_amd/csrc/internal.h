@@ -35,6 +35,7 @@ struct DecShape {
   int maxw = 8;
   int post_stride = 0, c2v_stride = 0;  // elements per slot, padded
   int wave_bytes = 0;                    // LDS bytes per wave
+  bool c2v_global = false;               // C2V in a global scratch (codewords too large for LDS)
 };
 
 // What the decode loop needs (device pointers + parameters)
@@ -47,6 +48,7 @@ struct DecCore {
   uint32_t max_iter;
   long long ibeta, icap;    // integer-element beta and cap (already truncated/wrapped)
   double fbeta, fcap;       // float-element beta and cap
+  void* c2v_scratch;        // [grid slots][c2v_stride] when the shape keeps C2V in global memory
 };
 
 struct KernelArgs {
@@ -59,6 +61,23 @@ struct KernelArgs {
   void* c2v;
   unsigned long long* tot_iter;
 };
+
+// n / d for n < 2^31 by a 64-bit multiply: q = (n * m) >> s with m = ceil(2^s / d),
+// s = 31 + ceil(log2 d); exact because m*d - 2^s < d <= 2^(s-31) (Granlund-Montgomery)
+struct FastDiv {
+  uint32_t m = 1, s = 0;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  FastDiv f;
+  f.s = 31 + l;
+  f.m = (uint32_t)(((1ull << f.s) + d - 1) / d);
+  return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+  return (uint32_t)(((uint64_t)n * f.m) >> f.s);
+}
 
 // Data bits + encoder (sim.hip, k_encode)
 struct EncArgs {
@@ -92,6 +111,8 @@ struct LlrArgs {
   void* out;                 // LLRs of the decoder's element type
   long long cw_stride, col_stride, base;  // output layout in elements
   unsigned long long* uncoded;            // device counter (added to), may be null
+  int has_mask;              // any puncture or filler bit
+  FastDiv div_cw, div_z;     // by hm*z and by z (set by the launcher)
 };
 
 // Error counting (sim.hip, k_count)
@@ -106,6 +127,7 @@ struct CountArgs {
   unsigned long long* counts;      // device [2]: bit errors, codeword errors (added to)
   uint32_t* cw_bit_err;            // device [num_cw] or null
   unsigned long long* cw_counter;  // device counter of codewords (added to), or null
+  FastDiv div_dz, div_z;           // by ndata*z and by z (set by the launcher)
 };
 
 // Layout of the row-pair int8 kernel (pair_i8.hip)
@@ -125,6 +147,8 @@ struct DecoderDev {
   PairLayout pair;
   uint32_t* d_pair_adr = nullptr;
   int kernel = FECSIM_KERNEL_AUTO;
+  void* d_c2v_scratch = nullptr;  // generic kernel, c2v_global shapes only
+  uint32_t max_grid = 0;          // blocks the generic kernel is launched with at most (0: no cap)
 };
 
 int pair_i8_shape(const DecoderDev& d);
@@ -133,6 +157,8 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t 
 int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why);
 void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
+// blocks of the generic kernel resident on the whole device for decoder d's shape
+int generic_resident_blocks(const DecoderDev& d);
 int encode_shape(int z, int mcol, EncArgs* a);
 hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);

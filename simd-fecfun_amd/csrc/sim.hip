@@ -182,9 +182,9 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
   unsigned long long err = 0;
   const uint32_t total = a.num_cw * per_cw;   // host splits launches below 2^31
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const uint32_t cw = t / per_cw;
+    const uint32_t cw = fdiv(t, a.div_cw);
     const uint32_t rem = t - cw * per_cw;
-    const uint32_t k = rem / (uint32_t)a.z;
+    const uint32_t k = fdiv(rem, a.div_z);
     const int r = (int)(rem - k * (uint32_t)a.z);
     const uint64_t p = ((uint64_t)(cw_base + a.cw_off + cw) * (uint64_t)a.hm + k) * (uint64_t)a.z + (uint64_t)r;
     float ns, nc;
@@ -202,9 +202,11 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
       float l = rx * a.factor;
       l = a.max_abs < l ? a.max_abs : l;
       l = -a.max_abs > l ? -a.max_abs : l;
-      const int mw = col * a.zw + (r >> 5);
-      if ((a.punct[mw] >> (r & 31)) & 1u) l = 0.0f;
-      if ((a.fill[mw] >> (r & 31)) & 1u) l = a.max_abs;
+      if (a.has_mask) {
+        const int mw = col * a.zw + (r >> 5);
+        if ((a.punct[mw] >> (r & 31)) & 1u) l = 0.0f;
+        if ((a.fill[mw] >> (r & 31)) & 1u) l = a.max_abs;
+      }
       out[(long long)gcw * a.cw_stride + (long long)col * a.col_stride + a.base + r] = to_elem<T>(l);
     }
   }
@@ -231,9 +233,9 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t cw_first = tile * a.cpb;
     for (uint32_t f = threadIdx.x; f < items; f += blockDim.x) {
-      const uint32_t lc = f / dz;
+      const uint32_t lc = fdiv(f, a.div_dz);
       const uint32_t rem = f - lc * dz;
-      const uint32_t c = rem / (uint32_t)a.z;
+      const uint32_t c = fdiv(rem, a.div_z);
       const int r = (int)(rem - c * (uint32_t)a.z);
       const uint64_t cw = cw_first + lc;
       if (cw >= a.num_cw) continue;
@@ -318,6 +320,8 @@ hipError_t launch_llr(int elem, const LlrArgs& a0, hipStream_t st) {
   const uint64_t per_cw = (uint64_t)a0.hm * a0.z;
   const uint32_t chunk = (uint32_t)((1ull << 30) / per_cw);
   LlrArgs a = a0;
+  a.div_cw = make_fastdiv((uint32_t)per_cw);
+  a.div_z = make_fastdiv((uint32_t)a0.z);
   for (uint64_t done = 0; done < a0.num_cw; done += chunk) {
     const uint32_t n = (uint32_t)((a0.num_cw - done) < chunk ? (a0.num_cw - done) : chunk);
     a.cw_off = a0.cw_off + done;
@@ -354,6 +358,8 @@ hipError_t launch_count(int elem, const CountArgs& a0, hipStream_t st) {
   a.cpb = (int)(2048 / dz);
   if (a.cpb < 1) a.cpb = 1;
   if (a.cpb > kCountMaxTile) a.cpb = kCountMaxTile;
+  a.div_dz = make_fastdiv(dz);
+  a.div_z = make_fastdiv((uint32_t)a.z);
   const uint64_t tiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
   uint64_t grid = tiles;
   const uint64_t cap = (uint64_t)resident_blocks((const void*)k_count<int8_t>, 256, 0) * 4;

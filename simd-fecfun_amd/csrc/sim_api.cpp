@@ -210,6 +210,7 @@ int llr_args(fecsim_henc* h, float sigma, float factor, float max_abs, uint64_t 
   a->out = llr;
   layout_strides(hh.z, hh.mcol, layout_p, &a->cw_stride, &a->col_stride, &a->base);
   a->uncoded = uncoded;
+  a->has_mask = hh.punc_bits > 0 || hh.fill_bits > 0;
   return FECSIM_OK;
 }
 

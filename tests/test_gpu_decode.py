@@ -228,3 +228,37 @@ def test_gpu_pair_kernel_selected_for_11ad(cuda):
     assert Decoder(h, "Vec32s_min", 10, 31.0, 1.0).kernel == L.KERNEL_GENERIC
     odd = codes.gen_structs("11n 1/2 z27", simd_size=512, elem_size=8)
     assert Decoder(Hdec.from_tables(odd.dec), "Vec64c_min", 10, 31.0, 1.0).kernel == L.KERNEL_GENERIC
+
+
+@pytest.mark.parametrize("case", [
+    # (matrix set, kind, max_iter, SNR dB, codewords): codewords whose C2V does not fit the
+    # wave's LDS slice keep it in a global scratch; 3000 x 11n z81 double exceeds the
+    # resident grid, so the kernel's stride loop runs several rounds
+    ("5GNR_n25344_k8448", "Vec64c_min", 20, 0.2 - 10 * np.log10(3.0), 6),   # Eb/N0 0.2 dB: 20 iterations
+    ("5GNR_n25344_k8448", "Vec64c_min", 20, 0.6 - 10 * np.log10(3.0), 6),   # Eb/N0 0.6 dB: converges
+    ("5GNR_n25344_k8448", "Vec32s_min", 20, 0.6 - 10 * np.log10(3.0), 3),
+    ("11n 1/2 z81", "Vec8d_min", 10, -0.5, 3000),
+], ids=lambda c: "%s-%s-%.2f-%d" % (c[0], c[1], c[3], c[4]))
+def test_gpu_global_c2v_codes_match_oracle(case, cuda):
+    """Large codes (5G NR base graph 1, Z=384, rate 1/3 with 2Z punctured columns):
+    posteriors, C2V and iterations against the oracle."""
+    ms, kind, mi, snr, ncw = case
+    _say("global c2v", case)
+    cs = codes.gen_structs(ms, simd_size=512, elem_size=8)
+    elem = O.ELEM_OF_CHAR[kind[kind.index("Vec") + 3:].lstrip("0123456789")[0]]
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    scale, mx = (2.0, 24.0) if elem in (O.I8, O.I16) else (1.0, 1e9)
+    llr, _ = rs.metrics(wgn, bits, snr, elem=elem, llr_scale=scale, max_abs=mx)
+    ref = O.decode_ideal(cs.dec, llr, elem=elem, max_iter=mi)
+    post, c2v, iters, tot = _gpu_decode(cs.dec, kind, llr, mi, 31.0, 1.0)
+    assert np.array_equal(iters, ref[2])
+    if elem in (O.F32, O.F64):
+        np.testing.assert_allclose(post, ref[0], rtol=1e-6, atol=1e-6)   # same op order: equal in practice
+        assert _bytes_eq(post, ref[0])
+    else:
+        assert _bytes_eq(post, ref[0])
+    assert _bytes_eq(c2v, ref[1])
+    assert tot == ref[3]
+    ce, _ = refsim.count_errors(cs, bits, post, elem=elem)
+    _say("  iterations", iters.mean(), "cw errors", ce, "of", ncw)
