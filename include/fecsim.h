@@ -95,6 +95,15 @@ int fecsim_decoder_kernel(const fecsim_decoder *d);
 
 /* element kind (FECSIM_I8..F64), mode (FECSIM_MIN/PROD), bytes per element */
 int fecsim_decoder_info(const fecsim_decoder *d, int *elem, int *mode, int *elem_bytes);
+
+/* Per-(iteration, layer) offsets: beta[max_iter][mrow] host floats, each converted as
+ * sim_struct.beta_offset is (float -> int truncation -> element width for integer
+ * kinds, decode_functions.cpp:60-61), used by iteration t, layer l in place of
+ * beta_offset.  NULL restores the constant offset.  No reference counterpart: this is
+ * how offsets learned with the trainable decoder below are run in the bit-exact
+ * decoders; with every entry equal to beta_offset the results are the plain
+ * decoder's.  A decoder with a table runs the generic kernel. */
+int fecsim_decoder_set_offsets(fecsim_decoder *d, const float *beta);
 void fecsim_decoder_free(fecsim_decoder *d);
 
 /* Replaces LDPCdecode_set() (src/LDPCDec/LDPCdecode.cpp:203-245): decodes num_cw
@@ -195,6 +204,14 @@ int fecsim_gen_llr(fecsim_henc *h, const fecsim_chan_params *p, uint64_t seed, u
 /* sigma and LLR factor the channel uses for p (host computation, gen_metrics.cpp:28-30) */
 int fecsim_chan_sigma(const fecsim_chan_params *p, float *sigma, float *factor);
 
+/* fecsim_gen_llr with a channel per codeword: sigma_factor is device [num_cw][2] floats
+ * (sigma, LLR factor) as fecsim_chan_sigma returns them; everything else as
+ * fecsim_gen_llr.  No reference counterpart (its SNR is per batch); used for batches
+ * whose codewords see different SNRs. */
+int fecsim_gen_llr_var(fecsim_henc *h, const float *sigma_factor, float max_abs_llr, uint64_t seed, uint64_t cw0,
+                       uint32_t num_cw, int col_words, const uint32_t *bits, int elem, int layout_p, void *llr,
+                       unsigned long long *uncoded_err, void *stream);
+
 /* Replaces LDPC_count_errors() (src/Top/LDPC_count_errors.cpp:40-136): hard decisions
  * (value < 0) of the data columns of `post` against `bits`; adds the bit errors to
  * counts[0] and the codeword errors (one per interleaved codeword slot r % num_par
@@ -238,6 +255,36 @@ int fecsim_sim_stats_get(fecsim_sim *s, fecsim_sim_stats *out);
 /* zeroes the counters (stream-ordered); the codeword index keeps advancing */
 int fecsim_sim_reset(fecsim_sim *s, void *stream);
 void fecsim_sim_free(fecsim_sim *s);
+
+/* ------------------------------------------------------------------------- */
+/* Differentiable decoder (no reference counterpart)                          */
+/* ------------------------------------------------------------------------- */
+
+/* Layered offset / normalized min-sum in float or double, unrolled for T iterations
+ * (no syndrome exit), with an offset beta[t][l] and a scale alpha[t][l] per
+ * (iteration, layer): C2V_j = sign * alpha * min(cap, min_{k != j} max(|q_k| - beta, 0)).
+ * With alpha = 1 and beta = beta_offset it is the Vec16f_min / Vec8d_min layer update
+ * (src/LDPCDec/decode_functions.cpp:11-103).  The adjoint recomputes each
+ * iteration from per-iteration checkpoints (posteriors and C2V) kept by the forward. */
+typedef struct fecsim_trainer fecsim_trainer;
+
+typedef struct {
+    uint32_t iters;      /* T */
+    uint32_t max_batch;  /* codewords per call at most (sizes the checkpoints) */
+    float cap;           /* row cap (max_row_metric) */
+    int elem;            /* FECSIM_F32 or FECSIM_F64 */
+} fecsim_train_params;
+
+int fecsim_trainer_create(const fecsim_hdec *h, const fecsim_train_params *p, fecsim_trainer **out);
+/* llr, post: device [batch][mcol*z]; alpha, beta: device [T][mrow]; all of elem's type */
+int fecsim_train_fwd(fecsim_trainer *t, uint32_t batch, const void *llr, const void *alpha, const void *beta,
+                     void *post, void *stream);
+/* Gradients of a loss L given dL/d(post) of the last forward (same batch, llr, alpha,
+ * beta): grad_alpha, grad_beta [T][mrow] (overwritten, summed over the batch),
+ * grad_llr [batch][mcol*z] or NULL. */
+int fecsim_train_bwd(fecsim_trainer *t, uint32_t batch, const void *grad_post, void *grad_alpha, void *grad_beta,
+                     void *grad_llr, void *stream);
+void fecsim_trainer_free(fecsim_trainer *t);
 
 #ifdef __cplusplus
 }

@@ -317,6 +317,10 @@ static double dbl_cap(float mrm) { return mrm < 1e-6f ? 1.0 - (double)mrm : (dou
 /* ideal layered decoder                                                     */
 /* ------------------------------------------------------------------------ */
 
+/* per-(iteration, layer) offsets replacing beta (converted the same way), or NULL */
+static const float *g_beta_tab = NULL;
+void orc_set_beta_table(const float *tab) { g_beta_tab = tab; }
+
 static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t max_iter, float mrm, float beta,
                          void *llr, void *c2v, uint32_t *iters, int use_tree) {
     const int bits = elem_bits(elem);
@@ -334,6 +338,7 @@ static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t 
             int e0 = 0;
             for (int l = 0; l < h->mrow; ++l) {
                 const int w = h->row_weights[l];
+                const int64_t bl = g_beta_tab ? int_beta(g_beta_tab[iter * h->mrow + l], bits) : b;
                 for (int r = 0; r < Z; ++r) {
                     int64_t Q[MAXW];
                     double A[MAXW], M[MAXW];
@@ -343,7 +348,7 @@ static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t 
                         const int e = e0 + j;
                         vidx[j] = h->row_active_cols[e] * Z + (h->row_offset[e] + r) % Z;
                         Q[j] = isat(L[vidx[j]] - R[e * Z + r], bits);
-                        int64_t a = iwrap(iwabs(Q[j], bits) - b, bits);
+                        int64_t a = iwrap(iwabs(Q[j], bits) - bl, bits);
                         A[j] = (double)(a > 0 ? a : 0);
                         sgn ^= (Q[j] < 0);
                     }
@@ -421,6 +426,7 @@ static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t 
                 int e0 = 0;                                                                                     \
                 for (int l = 0; l < h->mrow; ++l) {                                                             \
                     const int w = h->row_weights[l];                                                            \
+                    const FT bl = g_beta_tab ? (FT)g_beta_tab[iter * h->mrow + l] : b;                          \
                     for (int r = 0; r < Z; ++r) {                                                               \
                         FT Q[MAXW], A[MAXW], M[MAXW];                                                           \
                         int vidx[MAXW];                                                                         \
@@ -429,7 +435,7 @@ static int64_t ideal_int(const orc_hdec *h, int elem, uint32_t num_cw, uint32_t 
                             const int e = e0 + j;                                                               \
                             vidx[j] = h->row_active_cols[e] * Z + (h->row_offset[e] + r) % Z;                   \
                             Q[j] = L[vidx[j]] - R[e * Z + r];                                                   \
-                            FT a = FABS(Q[j]) - b;                                                              \
+                            FT a = FABS(Q[j]) - bl;                                                             \
                             A[j] = a > (FT)0 ? a : (FT)0; /* _mm_max_ps(a, 0) */                                \
                             sgn ^= (signbit(Q[j]) != 0);                                                        \
                         }                                                                                       \

@@ -58,6 +58,11 @@ int64_t orc_decode_ideal(const orc_hdec *h, int elem, int mode, uint32_t num_cw,
                          float max_row_metric, float beta_offset, void *llr, void *c2v, uint32_t *iters,
                          int use_tree);
 
+/* Per-(iteration, layer) offsets [max_iter][mrow] used by orc_decode_ideal in place of
+ * beta_offset (converted the same way), or NULL for the constant.  The table must
+ * cover every iteration of the next calls; it is read, not copied. */
+void orc_set_beta_table(const float *tab);
+
 /* Literal decoder on the reference layout: metrics [num_cw][mcol][(SPZ+2)*P] with
  * column c's data at element P + c*(SPZ+2)*P.  c2v out as in orc_decode_ideal. */
 int64_t orc_decode_literal(const orc_hdec *h, int elem, int mode, int P, uint32_t num_cw, uint32_t max_iter,
@@ -110,6 +115,19 @@ uint32_t orc_noise_llr(uint64_t seed, uint64_t cw0, uint32_t num_cw, const uint8
 /* the product's counter-based data bits (words layout as orc_gen_rand_databits) */
 void orc_gpu_databits(uint64_t seed, uint64_t cw0, uint32_t num_cw, int z, int mrow, int mcol,
                       const uint8_t *filler_mask, uint8_t *cw_bits);
+
+/* ---- differentiable offset/normalized min-sum (oracle/train.c) ---- */
+/* T iterations, no early exit; alpha/beta [T][mrow]; llr [num_cw][N] in: channel, out:
+ * posterior; c2v_out [num_cw][E] or NULL.  Returns 0, negative on error. */
+int orc_train_fwd_f32(const orc_hdec *h, uint32_t num_cw, uint32_t T, const float *alpha, const float *beta,
+                      float cap, float *llr, float *c2v_out);
+int orc_train_fwd_f64(const orc_hdec *h, uint32_t num_cw, uint32_t T, const double *alpha, const double *beta,
+                      double cap, double *llr, double *c2v_out);
+/* reverse mode of orc_train_fwd_f64: gpost = dL/d(posterior) [num_cw][N] -> summed
+ * galpha/gbeta [T][mrow] and gllr = dL/d(channel LLR) [num_cw][N] (may be NULL) */
+int orc_train_grad_f64(const orc_hdec *h, uint32_t num_cw, uint32_t T, const double *alpha, const double *beta,
+                       double cap, const double *llr, const double *gpost, double *galpha, double *gbeta,
+                       double *gllr);
 
 /* own float/double math used by the sum-product decoder (parity unpinned against
  * vectorclass's tanh/atanh; see the header comment of oracle.c) */

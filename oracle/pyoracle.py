@@ -71,6 +71,10 @@ def lib():
         L.orc_noise_llr.restype = u32
         L.orc_noise_llr.argtypes = [C.c_uint64, C.c_uint64, u32, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp, vp, C.c_int, vp]
         L.orc_gpu_databits.argtypes = [C.c_uint64, C.c_uint64, u32, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.orc_set_beta_table.argtypes = [vp]
+        L.orc_train_fwd_f32.argtypes = [C.POINTER(Hdec), u32, u32, vp, vp, C.c_float, vp, vp]
+        L.orc_train_fwd_f64.argtypes = [C.POINTER(Hdec), u32, u32, vp, vp, C.c_double, vp, vp]
+        L.orc_train_grad_f64.argtypes = [C.POINTER(Hdec), u32, u32, vp, vp, C.c_double, vp, vp, vp, vp, vp]
         for f in ("orc_tanhf", "orc_atanhf"):
             getattr(L, f).restype = C.c_float
             getattr(L, f).argtypes = [C.c_float]
@@ -115,8 +119,17 @@ class OracleHdec:
 
 
 def decode_ideal(dec, llr, elem=I8, mode=MIN, max_iter=10, max_row_metric=31.0, beta=1.0, use_tree=False,
-                 want_c2v=True):
-    """llr: [num_cw, N] array of the element type.  Returns (post, c2v, iters, tot)."""
+                 want_c2v=True, beta_table=None):
+    """llr: [num_cw, N] array of the element type.  Returns (post, c2v, iters, tot).
+    beta_table: optional [max_iter, mrow] per-(iteration, layer) offsets."""
+    if beta_table is not None:
+        tab = np.ascontiguousarray(beta_table, dtype=np.float32)
+        assert tab.shape == (max_iter, dec.mrow)
+        lib().orc_set_beta_table(tab.ctypes.data)
+        try:
+            return decode_ideal(dec, llr, elem, mode, max_iter, max_row_metric, beta, use_tree, want_c2v)
+        finally:
+            lib().orc_set_beta_table(None)
     h = OracleHdec(dec)
     post = np.ascontiguousarray(llr, dtype=NP[elem]).copy()
     ncw = post.shape[0]
@@ -193,3 +206,35 @@ def noise_llr(seed, cw0, bits, z, mcol, sigma, factor, max_abs, punct=None, fill
     err = lib().orc_noise_llr(seed, cw0, ncw, _p(bits), z, mcol, sigma, factor, max_abs, _p(pu), _p(fi), elem,
                               _p(out))
     return out, err
+
+
+def train_fwd(dec, llr, alpha, beta, cap=1e30, dtype=np.float32, want_c2v=False):
+    """Differentiable min-sum forward (T = alpha.shape[0] iterations).  Returns (post, c2v)."""
+    h = OracleHdec(dec)
+    post = np.ascontiguousarray(llr, dtype=dtype).copy()
+    a = np.ascontiguousarray(alpha, dtype=dtype)
+    b = np.ascontiguousarray(beta, dtype=dtype)
+    T = a.shape[0]
+    ncw = post.shape[0]
+    c2v = np.zeros((ncw, dec.total_hc_ones * dec.z), dtype) if want_c2v else None
+    f = lib().orc_train_fwd_f32 if dtype == np.float32 else lib().orc_train_fwd_f64
+    rc = f(h.ref, ncw, T, _p(a), _p(b), cap, _p(post), _p(c2v))
+    if rc:
+        raise RuntimeError("orc_train_fwd error %d" % rc)
+    return post, c2v
+
+
+def train_grad(dec, llr, alpha, beta, gpost, cap=1e30):
+    """Reverse mode of train_fwd in double: (galpha, gbeta, gllr)."""
+    h = OracleHdec(dec)
+    x = np.ascontiguousarray(llr, dtype=np.float64)
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    g = np.ascontiguousarray(gpost, dtype=np.float64)
+    T = a.shape[0]
+    ga, gb = np.zeros_like(a), np.zeros_like(b)
+    gl = np.zeros_like(x)
+    rc = lib().orc_train_grad_f64(h.ref, x.shape[0], T, _p(a), _p(b), cap, _p(x), _p(g), _p(ga), _p(gb), _p(gl))
+    if rc:
+        raise RuntimeError("orc_train_grad_f64 error %d" % rc)
+    return ga, gb, gl

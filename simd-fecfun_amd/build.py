@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libfecsim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["decode.hip", "pair_i8.hip", "sim.hip", "api.cpp", "sim_api.cpp"]
+SOURCES = ["decode.hip", "pair_i8.hip", "sim.hip", "train.hip", "api.cpp", "sim_api.cpp", "train_api.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -12,10 +12,6 @@
 
 using namespace fecsim;
 
-struct fecsim_hdec {
-  HdecHost h;
-  int32_t len_f2b = 0, len_b2f = 0;
-};
 
 namespace fecsim {
 
@@ -252,8 +248,30 @@ int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
 
 int fecsim_decoder_kernel(const fecsim_decoder* d) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
-  if (d->d.kernel == FECSIM_KERNEL_GENERIC || d->d.pair_shape < 0) return FECSIM_KERNEL_GENERIC;
+  if (d->d.kernel == FECSIM_KERNEL_GENERIC || d->d.pair_shape < 0 || d->d.d_beta_tab) return FECSIM_KERNEL_GENERIC;
   return FECSIM_KERNEL_PAIR;
+}
+
+int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
+  if (!d) return fail(FECSIM_E_INVAL, "null argument");
+  DecoderDev& dv = d->d;
+  if (!beta) {
+    if (dv.d_beta_tab) (void)hipFree(dv.d_beta_tab);
+    dv.d_beta_tab = nullptr;
+    return FECSIM_OK;
+  }
+  const size_t n = (size_t)dv.params.max_iter * dv.h.mrow;
+  std::vector<long long> tab;
+  offsets_table(dv, beta, n, &tab);
+  void* p = dv.d_beta_tab;
+  hipError_t he = p ? hipSuccess : hipMalloc(&p, n * sizeof(long long));
+  if (he == hipSuccess) he = hipMemcpy(p, tab.data(), n * sizeof(long long), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    if (p && !dv.d_beta_tab) (void)hipFree(p);
+    return hip_fail(he, "offset table to device");
+  }
+  dv.d_beta_tab = p;
+  return FECSIM_OK;
 }
 
 int fecsim_decoder_info(const fecsim_decoder* d, int* elem, int* mode, int* elem_bytes) {
@@ -270,6 +288,7 @@ void fecsim_decoder_free(fecsim_decoder* d) {
   if (d->d.d_layer_w) (void)hipFree(d->d.d_layer_w);
   if (d->d.d_pair_adr) (void)hipFree(d->d.d_pair_adr);
   if (d->d.d_c2v_scratch) (void)hipFree(d->d.d_c2v_scratch);
+  if (d->d.d_beta_tab) (void)hipFree(d->d.d_beta_tab);
   delete d;
 }
 
@@ -289,7 +308,7 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   a.iters = iters;
   a.c2v = c2v;
   a.tot_iter = tot_iter;
-  const bool pair = layout_p == 0 && d.pair_shape >= 0 && d.kernel != FECSIM_KERNEL_GENERIC;
+  const bool pair = layout_p == 0 && d.pair_shape >= 0 && d.kernel != FECSIM_KERNEL_GENERIC && !d.d_beta_tab;
   hipError_t he = pair ? launch_pair_i8(d, a, (hipStream_t)stream) : launch_decode(d, a, (hipStream_t)stream);
   if (he != hipSuccess) return hip_fail(he, "decode launch");
   return FECSIM_OK;

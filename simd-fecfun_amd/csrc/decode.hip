@@ -1,6 +1,8 @@
 // Generic layered decoder kernel: any .dat table, any element kind, dense or the
 // reference's padded memory layout (see decode_core.h for the semantics and the
 // wave/LDS mapping).
+#include <cstring>
+
 #include "decode_core.h"
 
 namespace fecsim {
@@ -135,6 +137,7 @@ void fill_core(const DecoderDev& dv, DecCore* c) {
   c->wave_bytes = dv.shape.wave_bytes;
   c->max_iter = dv.params.max_iter;
   c->c2v_scratch = dv.d_c2v_scratch;
+  c->beta_tab = dv.d_beta_tab;
   const float mrm = dv.params.max_row_metric, beta = dv.params.beta_offset;
   c->ibeta = c->icap = 0;
   c->fbeta = c->fcap = 0.0;
@@ -154,6 +157,18 @@ void fill_core(const DecoderDev& dv, DecCore* c) {
   } else {
     c->fbeta = (double)beta;
     c->fcap = mrm < 1e-6f ? 1.0 - (double)mrm : (double)mrm;
+  }
+}
+
+void offsets_table(const DecoderDev& dv, const float* beta, size_t n, std::vector<long long>* out) {
+  out->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (dv.elem <= FECSIM_I32) {
+      (*out)[i] = wrap_to((long long)(int)beta[i], 8 * dv.elem_bytes);   // as beta_offset
+    } else {
+      const double v = (double)beta[i];
+      memcpy(&(*out)[i], &v, sizeof(v));
+    }
   }
 }
 

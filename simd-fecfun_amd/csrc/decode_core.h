@@ -206,11 +206,17 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
       int e0 = 0;
       for (int l = 0; l < a.mrow; ++l) {
         const int w = a.layer_w[l];
+        S fb = fbeta;
+        long long ib = a.ibeta;
+        if (a.beta_tab) {   // per-(iteration, layer) offsets
+          if constexpr (P::kFloat) fb = (S)reinterpret_cast<const double*>(a.beta_tab)[iter * a.mrow + l];
+          else ib = reinterpret_cast<const long long*>(a.beta_tab)[iter * a.mrow + l];
+        }
         for (int k = 0; k < a.rows_per_lane; ++k) {
           const int r = lr + k * L;
           if (r < Z) {
-            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, fbeta, fcap);
-            else row_int<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, (typename P::C)a.ibeta, (typename P::C)a.icap);
+            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, fb, fcap);
+            else row_int<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
           }
         }
         e0 += w;

@@ -49,6 +49,7 @@ struct DecCore {
   long long ibeta, icap;    // integer-element beta and cap (already truncated/wrapped)
   double fbeta, fcap;       // float-element beta and cap
   void* c2v_scratch;        // [grid slots][c2v_stride] when the shape keeps C2V in global memory
+  const void* beta_tab;     // [max_iter][mrow] offsets (long long / double) or null
 };
 
 struct KernelArgs {
@@ -113,6 +114,7 @@ struct LlrArgs {
   unsigned long long* uncoded;            // device counter (added to), may be null
   int has_mask;              // any puncture or filler bit
   FastDiv div_cw, div_z;     // by hm*z and by z (set by the launcher)
+  const float* sf;           // per-codeword (sigma, factor) pairs [num_cw][2] replacing sigma/factor, or null
 };
 
 // Error counting (sim.hip, k_count)
@@ -128,6 +130,32 @@ struct CountArgs {
   uint32_t* cw_bit_err;            // device [num_cw] or null
   unsigned long long* cw_counter;  // device counter of codewords (added to), or null
   FastDiv div_dz, div_z;           // by ndata*z and by z (set by the launcher)
+};
+
+// Differentiable min-sum forward / adjoint (train.hip); F = float or double
+struct TrainArgs {
+  const uint32_t* edges;     // decoder edge table ((col*z) << 16 | shift)
+  const int32_t* layer_w;
+  int mrow, z, n, e, n_pad;  // n_pad: LDS elements per posterior array
+  int iters;                 // T
+  int waves;                 // codewords (waves) per block
+  int is_double, global_p;   // element type; posteriors in global scratch instead of LDS
+  double cap;
+  uint64_t batch;
+  const void* llr;           // [batch][n]
+  const void* alpha;         // [T][mrow]
+  const void* beta;          // [T][mrow]
+  void* post;                // [batch][n]
+  void* ckP;                 // [batch][T][n] posteriors at the start of every iteration
+  void* ckC;                 // [batch][T+1][e] C2V at the start of every iteration (+ final)
+  void* qs;                  // [batch][e] V2C messages of the iteration being reversed
+  void* gc;                  // [batch][e] adjoint of the C2V messages
+  void* partial;             // [batch][T][mrow][2] per-codeword parameter gradients
+  void* gscratch;            // [batch][2][n] posteriors + adjoints when global_p
+  const void* grad_post;     // [batch][n]
+  void* grad_alpha;          // [T][mrow]
+  void* grad_beta;           // [T][mrow]
+  void* grad_llr;            // [batch][n] or null
 };
 
 // Layout of the row-pair int8 kernel (pair_i8.hip)
@@ -149,6 +177,7 @@ struct DecoderDev {
   int kernel = FECSIM_KERNEL_AUTO;
   void* d_c2v_scratch = nullptr;  // generic kernel, c2v_global shapes only
   uint32_t max_grid = 0;          // blocks the generic kernel is launched with at most (0: no cap)
+  void* d_beta_tab = nullptr;     // per-(iteration, layer) offsets: long long (int kinds) or double
 };
 
 int pair_i8_shape(const DecoderDev& d);
@@ -159,6 +188,11 @@ void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 // blocks of the generic kernel resident on the whole device for decoder d's shape
 int generic_resident_blocks(const DecoderDev& d);
+// device form of a per-(iteration, layer) offset table (long long for integer kinds,
+// the bits of a double for float kinds)
+void offsets_table(const DecoderDev& d, const float* beta, size_t n, std::vector<long long>* out);
+// forward (bwd = 0) or adjoint + parameter-gradient sum (bwd = 1) of the trainable decoder
+hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t stream);
 int encode_shape(int z, int mcol, EncArgs* a);
 hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);
@@ -191,4 +225,9 @@ inline void layout_strides(int z, int mcol, int layout_p, long long* cw_stride, 
 
 struct fecsim_decoder {
   fecsim::DecoderDev d;
+};
+
+struct fecsim_hdec {
+  fecsim::HdecHost h;
+  int32_t len_f2b = 0, len_b2f = 0;
 };

@@ -190,6 +190,11 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
     float ns, nc;
     noise_pair(a.seed, p, &ns, &nc);
     const uint64_t gcw = a.cw_off + cw;
+    float sigma = a.sigma, factor = a.factor;
+    if (a.sf) {
+      sigma = a.sf[2 * gcw];
+      factor = a.sf[2 * gcw + 1];
+    }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int col = 2 * (int)k + h;
@@ -197,9 +202,9 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
       const uint32_t word = a.bits[(gcw * a.mcol + col) * a.col_words + (r >> 5)];
       const uint32_t bit = (word >> (r & 31)) & 1u;
       const float orig = bit ? -1.0f : 1.0f;
-      const float rx = __builtin_fmaf(h ? nc : ns, a.sigma, orig);
+      const float rx = __builtin_fmaf(h ? nc : ns, sigma, orig);
       err += (__float_as_uint(orig) ^ __float_as_uint(rx)) >> 31;
-      float l = rx * a.factor;
+      float l = rx * factor;
       l = a.max_abs < l ? a.max_abs : l;
       l = -a.max_abs > l ? -a.max_abs : l;
       if (a.has_mask) {

@@ -377,6 +377,23 @@ int fecsim_gen_llr(fecsim_henc* h, const fecsim_chan_params* p, uint64_t seed, u
   return e == hipSuccess ? FECSIM_OK : hip_fail(e, "llr launch");
 }
 
+int fecsim_gen_llr_var(fecsim_henc* h, const float* sigma_factor, float max_abs_llr, uint64_t seed, uint64_t cw0,
+                       uint32_t num_cw, int col_words, const uint32_t* bits, int elem, int layout_p, void* llr,
+                       unsigned long long* uncoded_err, void* stream) {
+  if (!h || ((!bits || !llr || !sigma_factor) && num_cw)) return fail(FECSIM_E_INVAL, "null argument");
+  if (col_words < h->h.zw || col_words > 4096) return fail(FECSIM_E_INVAL, "col_words below ceil(z/32)");
+  if (!elem_ok(elem)) return fail(FECSIM_E_INVAL, "unknown element kind");
+  if (layout_p < 0 || layout_p > 4096) return fail(FECSIM_E_INVAL, "layout_p outside 0..4096");
+  if (!num_cw) return FECSIM_OK;
+  int rc = ensure_device(h);
+  if (rc) return rc;
+  LlrArgs a;
+  llr_args(h, 0.0f, 0.0f, max_abs_llr, seed, cw0, nullptr, num_cw, col_words, bits, layout_p, llr, uncoded_err, &a);
+  a.sf = sigma_factor;
+  const hipError_t e = launch_llr(elem, a, (hipStream_t)stream);
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "llr launch");
+}
+
 int fecsim_count_errors(fecsim_henc* h, uint32_t num_cw, int col_words, const uint32_t* bits, int elem, int layout_p,
                         const void* post, unsigned long long* counts, uint32_t* cw_bit_err, void* stream) {
   if (!h || !counts || ((!bits || !post) && num_cw)) return fail(FECSIM_E_INVAL, "null argument");

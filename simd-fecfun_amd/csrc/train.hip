@@ -1,0 +1,286 @@
+// Differentiable layered offset / normalized min-sum decoder: forward and adjoint.
+//
+// No counterpart in the reference.  The forward is the reference's float min-sum
+// layer update (src/LDPCDec/decode_functions.cpp:11-103: A_j = max(|q_j| - beta, 0),
+// exclusive minimum capped at max_row_metric, sign of the product of the other
+// signs) run for a fixed T iterations (no syndrome exit) with an offset beta[t][l]
+// and a scale alpha[t][l] per (iteration, layer):  M_j = alpha * min(cap, min_{k!=j} A_k).
+// With alpha = 1 and a constant beta it equals the reference decoder on codewords
+// that need all T iterations.
+//
+// Adjoint: reverse sweep over iterations; iteration t restarts from its checkpoint
+// (posteriors P_t and C2V C_t stored by the forward), recomputes its V2C messages
+// q (kept in a scratch array), then walks the layers backwards:
+//   g_out_j = gP[v_j] + gC[e_j]           (C2V_j feeds the posterior and iteration t+1)
+//   g_m_j   = sign_j * g_out_j,  g_alpha += g_m_j * min(cap, x_j),  g_A[src_j] += alpha * g_m_j (x_j < cap)
+//   g_q_j   = gP[v_j] + [|q_j| > beta] * sign(q_j) * g_A_j,  g_beta -= [|q_j| > beta] * g_A_j
+//   gP[v_j] <- g_q_j,  gC[e_j] <- -g_q_j    (gradient w.r.t. the C2V entering iteration t)
+// where x_j is the exclusive minimum and src_j the edge it came from.  Parameter
+// gradients are written per codeword and summed by k_train_sum (deterministic).
+//
+// One wave per codeword, lane owns rows lane, lane+64, ... of every layer.
+#include <hip/hip_runtime.h>
+
+#include "decode_core.h"
+#include "internal.h"
+
+namespace fecsim {
+
+namespace {
+
+constexpr int kTrainMaxW = 32;
+
+// IEEE sign bit as 0/1 (-0.0 counts as negative, like the reference's sign_combine)
+__device__ __forceinline__ int sbit(float x) { return (int)(__float_as_uint(x) >> 31); }
+__device__ __forceinline__ int sbit(double x) { return (int)((unsigned long long)__double_as_longlong(x) >> 63); }
+
+template <typename F>
+struct TrainView {
+  F* P;    // posteriors (LDS or global)
+  F* gP;   // their adjoints (LDS or global)
+};
+
+template <typename F, bool GP>
+__device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t cw, char* smem, int wave) {
+  TrainView<F> v;
+  if constexpr (GP) {
+    v.P = reinterpret_cast<F*>(a.gscratch) + cw * (uint64_t)(2 * a.n);
+    v.gP = v.P + a.n;
+  } else {
+    v.P = reinterpret_cast<F*>(smem) + (size_t)wave * 2 * a.n_pad;
+    v.gP = v.P + a.n_pad;
+  }
+  return v;
+}
+
+// one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q
+template <typename F>
+__device__ __forceinline__ void fwd_row(const TrainArgs& a, F* P, const F* Cold, F* Cnew, F* Qs, int e0, int w,
+                                        int r, F al, F be, F cap) {
+  const int Z = a.z;
+  F q[kTrainMaxW];
+  int vi[kTrainMaxW];
+  F m1 = F(INFINITY), m2 = F(INFINITY);
+  int im = -1, sg = 0;
+#pragma unroll
+  for (int j = 0; j < kTrainMaxW; ++j) {
+    if (j < w) {
+      const uint32_t ed = a.edges[e0 + j];
+      int v = (int)(ed & 0xffffu) + r;
+      v = v >= Z ? v - Z : v;
+      vi[j] = v + (int)(ed >> 16);
+      const int ce = (e0 + j) * Z + r;
+      q[j] = P[vi[j]] - Cold[ce];
+      if (Qs) Qs[ce] = q[j];
+      const F x = fabs(q[j]) - be;
+      const F A = x > F(0) ? x : F(0);
+      sg ^= sbit(q[j]);
+      if (A < m1) { m2 = m1; m1 = A; im = j; }
+      else if (A < m2) m2 = A;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kTrainMaxW; ++j) {
+    if (j < w) {
+      const F x = (j == im) ? m2 : m1;
+      const F m = al * (cap < x ? cap : x);
+      const bool neg = sg ^ sbit(q[j]);
+      const F rn = neg ? -m : m;
+      P[vi[j]] = q[j] + rn;
+      if (Cnew) Cnew[(e0 + j) * Z + r] = rn;
+    }
+  }
+}
+
+template <typename F>
+__device__ __forceinline__ F wave_sum(F v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename F, bool GP>
+__global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
+  HIP_DYNAMIC_SHARED(char, smem);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t cw = (uint64_t)blockIdx.x * a.waves + wave;
+  if (cw >= a.batch) return;   // whole wave
+  const TrainView<F> tv = train_view<F, GP>(a, cw, smem, wave);
+  F* P = tv.P;
+  const int N = a.n, E = a.e, Z = a.z, T = a.iters;
+  const F* llr = reinterpret_cast<const F*>(a.llr) + cw * (uint64_t)N;
+  F* ckP = reinterpret_cast<F*>(a.ckP) + cw * (uint64_t)T * N;
+  F* ckC = reinterpret_cast<F*>(a.ckC) + cw * (uint64_t)(T + 1) * E;
+  const F* alpha = reinterpret_cast<const F*>(a.alpha);
+  const F* beta = reinterpret_cast<const F*>(a.beta);
+  const F cap = (F)a.cap;
+  for (int i = lane; i < N; i += 64) P[i] = llr[i];
+  for (int i = lane; i < E; i += 64) ckC[i] = F(0);
+  wave_sync();
+  for (int t = 0; t < T; ++t) {
+    F* cp = ckP + (uint64_t)t * N;
+    for (int i = lane; i < N; i += 64) cp[i] = P[i];
+    const F* Cold = ckC + (uint64_t)t * E;
+    F* Cnew = ckC + (uint64_t)(t + 1) * E;
+    int e0 = 0;
+    for (int l = 0; l < a.mrow; ++l) {
+      const int w = a.layer_w[l];
+      const F al = alpha[t * a.mrow + l], be = beta[t * a.mrow + l];
+      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, P, Cold, Cnew, (F*)nullptr, e0, w, r, al, be, cap);
+      e0 += w;
+      wave_sync();
+    }
+  }
+  F* out = reinterpret_cast<F*>(a.post) + cw * (uint64_t)N;
+  for (int i = lane; i < N; i += 64) out[i] = P[i];
+}
+
+template <typename F, bool GP>
+__global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
+  HIP_DYNAMIC_SHARED(char, smem);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t cw = (uint64_t)blockIdx.x * a.waves + wave;
+  if (cw >= a.batch) return;   // whole wave
+  const TrainView<F> tv = train_view<F, GP>(a, cw, smem, wave);
+  F* P = tv.P;
+  F* gP = tv.gP;
+  const int N = a.n, E = a.e, Z = a.z, T = a.iters, Lr = a.mrow;
+  const F* ckP = reinterpret_cast<const F*>(a.ckP) + cw * (uint64_t)T * N;
+  const F* ckC = reinterpret_cast<const F*>(a.ckC) + cw * (uint64_t)(T + 1) * E;
+  F* Qs = reinterpret_cast<F*>(a.qs) + cw * (uint64_t)E;
+  F* gC = reinterpret_cast<F*>(a.gc) + cw * (uint64_t)E;
+  F* part = reinterpret_cast<F*>(a.partial) + cw * (uint64_t)T * Lr * 2;
+  const F* alpha = reinterpret_cast<const F*>(a.alpha);
+  const F* beta = reinterpret_cast<const F*>(a.beta);
+  const F cap = (F)a.cap;
+  const F* gpost = reinterpret_cast<const F*>(a.grad_post) + cw * (uint64_t)N;
+  for (int i = lane; i < N; i += 64) gP[i] = gpost[i];
+  for (int i = lane; i < E; i += 64) gC[i] = F(0);
+  wave_sync();
+  for (int t = T - 1; t >= 0; --t) {
+    const F* cp = ckP + (uint64_t)t * N;
+    for (int i = lane; i < N; i += 64) P[i] = cp[i];
+    const F* Cold = ckC + (uint64_t)t * E;
+    wave_sync();
+    int e0 = 0;
+    for (int l = 0; l < Lr; ++l) {   // recompute, keeping q
+      const int w = a.layer_w[l];
+      const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
+      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, P, Cold, (F*)nullptr, Qs, e0, w, r, al, be, cap);
+      e0 += w;
+      wave_sync();
+    }
+    for (int l = Lr - 1; l >= 0; --l) {
+      const int w = a.layer_w[l];
+      e0 -= w;
+      const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
+      F ga = F(0), gb = F(0);
+      for (int r = lane; r < Z; r += 64) {
+        F q[kTrainMaxW], gA[kTrainMaxW], gq[kTrainMaxW];
+        int vi[kTrainMaxW];
+        F m1 = F(INFINITY), m2 = F(INFINITY);
+        int im = -1, im2 = -1, sg = 0;
+#pragma unroll
+        for (int j = 0; j < kTrainMaxW; ++j) {
+          gA[j] = F(0);
+          if (j < w) {
+            const uint32_t ed = a.edges[e0 + j];
+            int v = (int)(ed & 0xffffu) + r;
+            v = v >= Z ? v - Z : v;
+            vi[j] = v + (int)(ed >> 16);
+            q[j] = Qs[(e0 + j) * Z + r];
+            const F x = fabs(q[j]) - be;
+            const F A = x > F(0) ? x : F(0);
+            sg ^= sbit(q[j]);
+            if (A < m1) { m2 = m1; im2 = im; m1 = A; im = j; }
+            else if (A < m2) { m2 = A; im2 = j; }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kTrainMaxW; ++j) {
+          if (j < w) {
+            const int ce = (e0 + j) * Z + r;
+            const F x = (j == im) ? m2 : m1;
+            const int src = (j == im) ? im2 : im;
+            const bool capped = cap < x;
+            const F mc = capped ? cap : x;
+            const F s = (sg ^ sbit(q[j])) ? F(-1) : F(1);
+            const F gv = gP[vi[j]];
+            const F gm = s * (gv + gC[ce]);
+            gq[j] = gv;
+            ga += gm * mc;
+            if (!capped) {
+#pragma unroll
+              for (int k = 0; k < kTrainMaxW; ++k)
+                if (k == src) gA[k] += al * gm;
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kTrainMaxW; ++j) {
+          if (j < w) {
+            if (fabs(q[j]) - be > F(0)) {
+              gq[j] += sbit(q[j]) ? -gA[j] : gA[j];
+              gb -= gA[j];
+            }
+            gP[vi[j]] = gq[j];
+            gC[(e0 + j) * Z + r] = -gq[j];
+          }
+        }
+      }
+      wave_sync();
+      ga = wave_sum(ga);
+      gb = wave_sum(gb);
+      if (lane == 0) {
+        part[(t * Lr + l) * 2 + 0] = ga;
+        part[(t * Lr + l) * 2 + 1] = gb;
+      }
+    }
+  }
+  if (a.grad_llr) {
+    F* gl = reinterpret_cast<F*>(a.grad_llr) + cw * (uint64_t)N;
+    for (int i = lane; i < N; i += 64) gl[i] = gP[i];
+  }
+}
+
+// out[k] = sum over codewords of partial[cw][k], k < T*mrow*2; block k sums one entry
+template <typename F>
+__global__ void __launch_bounds__(256) k_train_sum(const F* partial, uint32_t batch, int per_cw, F* galpha, F* gbeta) {
+  __shared__ F red[4];
+  const int k = blockIdx.x;
+  F s = F(0);
+  for (uint32_t c = threadIdx.x; c < batch; c += blockDim.x) s += partial[(uint64_t)c * per_cw + k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    F t = F(0);
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+    if (k & 1) gbeta[k >> 1] = t;
+    else galpha[k >> 1] = t;
+  }
+}
+
+template <typename F, bool GP>
+hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
+  const uint64_t blocks = (a.batch + a.waves - 1) / a.waves;
+  const size_t lds = GP ? 0 : (size_t)a.waves * 2 * a.n_pad * sizeof(F);
+  if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  else hipLaunchKernelGGL((k_train_fwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !bwd) return e;
+  const int per_cw = a.iters * a.mrow * 2;
+  hipLaunchKernelGGL(k_train_sum<F>, dim3(per_cw), dim3(256), 0, st, reinterpret_cast<const F*>(a.partial),
+                     (uint32_t)a.batch, per_cw, reinterpret_cast<F*>(a.grad_alpha), reinterpret_cast<F*>(a.grad_beta));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t st) {
+  if (a.batch == 0) return hipSuccess;
+  if (a.is_double) return a.global_p ? launch_t<double, true>(a, bwd, st) : launch_t<double, false>(a, bwd, st);
+  return a.global_p ? launch_t<float, true>(a, bwd, st) : launch_t<float, false>(a, bwd, st);
+}
+
+}  // namespace fecsim

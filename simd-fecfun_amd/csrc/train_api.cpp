@@ -1,0 +1,158 @@
+// C ABI of the differentiable decoder (declared in include/fecsim.h).
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace fecsim;
+
+struct fecsim_trainer {
+  HdecHost h;
+  fecsim_train_params p{};
+  int elem_bytes = 4;
+  int waves = 4, global_p = 0, n_pad = 0;
+  uint32_t* d_edges = nullptr;
+  int32_t* d_layer_w = nullptr;
+  void* ckP = nullptr;
+  void* ckC = nullptr;
+  void* qs = nullptr;
+  void* gc = nullptr;
+  void* partial = nullptr;
+  void* gscratch = nullptr;
+  // arguments of the last forward, reused by the adjoint
+  uint32_t last_batch = 0;
+  const void* alpha = nullptr;
+  const void* beta = nullptr;
+};
+
+namespace {
+
+constexpr int kTrainMaxW = 32;
+
+void free_all(fecsim_trainer* t) {
+  (void)hipFree(t->d_edges);
+  (void)hipFree(t->d_layer_w);
+  (void)hipFree(t->ckP);
+  (void)hipFree(t->ckC);
+  (void)hipFree(t->qs);
+  (void)hipFree(t->gc);
+  (void)hipFree(t->partial);
+  (void)hipFree(t->gscratch);
+}
+
+TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
+  TrainArgs a{};
+  const HdecHost& h = t->h;
+  a.edges = t->d_edges;
+  a.layer_w = t->d_layer_w;
+  a.mrow = h.mrow;
+  a.z = h.z;
+  a.n = h.mcol * h.z;
+  a.e = h.total_hc_ones * h.z;
+  a.n_pad = t->n_pad;
+  a.iters = (int)t->p.iters;
+  a.waves = t->waves;
+  a.is_double = t->p.elem == FECSIM_F64;
+  a.global_p = t->global_p;
+  a.cap = (double)t->p.cap;
+  a.batch = batch;
+  a.ckP = t->ckP;
+  a.ckC = t->ckC;
+  a.qs = t->qs;
+  a.gc = t->gc;
+  a.partial = t->partial;
+  a.gscratch = t->gscratch;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, fecsim_trainer** out) {
+  if (!hd || !p || !out) return fail(FECSIM_E_INVAL, "null argument");
+  if (p->elem != FECSIM_F32 && p->elem != FECSIM_F64) return fail(FECSIM_E_INVAL, "elem must be FECSIM_F32 or FECSIM_F64");
+  if (p->iters == 0 || p->iters > 1000 || p->max_batch == 0) return fail(FECSIM_E_INVAL, "need 1..1000 iterations and max_batch >= 1");
+  const HdecHost& h = hd->h;
+  if (h.max_row_weight > kTrainMaxW) return fail(FECSIM_E_UNSUPPORTED, "row weight above 32");
+  fecsim_trainer* t = new (std::nothrow) fecsim_trainer();
+  if (!t) return fail(FECSIM_E_NOMEM, "host allocation");
+  t->h = h;
+  t->p = *p;
+  t->elem_bytes = p->elem == FECSIM_F64 ? 8 : 4;
+  const size_t N = (size_t)h.mcol * h.z, E = (size_t)h.total_hc_ones * h.z, T = p->iters, B = p->max_batch;
+  t->n_pad = (int)((N + 3) / 4 * 4);
+  const size_t wave_bytes = 2 * (size_t)t->n_pad * t->elem_bytes;
+  if (wave_bytes <= 65536) {
+    t->waves = (int)(65536 / wave_bytes);
+    if (t->waves > 4) t->waves = 4;
+  } else {
+    t->global_p = 1;
+    t->waves = 4;
+  }
+  std::vector<uint32_t> edges(h.total_hc_ones);
+  for (int e = 0; e < h.total_hc_ones; ++e)
+    edges[e] = ((uint32_t)(h.row_active_cols[e] * h.z) << 16) | (uint32_t)h.row_offset[e];
+  const size_t eb = t->elem_bytes;
+  hipError_t he = hipMalloc((void**)&t->d_edges, sizeof(uint32_t) * edges.size());
+  if (he == hipSuccess) he = hipMalloc((void**)&t->d_layer_w, sizeof(int32_t) * h.mrow);
+  if (he == hipSuccess) he = hipMemcpy(t->d_edges, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(t->d_layer_w, h.row_weights.data(), sizeof(int32_t) * h.mrow, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMalloc(&t->ckP, B * T * N * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->ckC, B * (T + 1) * E * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->qs, B * E * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->gc, B * E * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->partial, B * T * h.mrow * 2 * eb);
+  if (he == hipSuccess && t->global_p) he = hipMalloc(&t->gscratch, B * 2 * N * eb);
+  if (he != hipSuccess) {
+    free_all(t);
+    delete t;
+    return hip_fail(he, "trainer buffers");
+  }
+  *out = t;
+  return FECSIM_OK;
+}
+
+int fecsim_train_fwd(fecsim_trainer* t, uint32_t batch, const void* llr, const void* alpha, const void* beta,
+                     void* post, void* stream) {
+  if (!t || ((!llr || !alpha || !beta || !post) && batch)) return fail(FECSIM_E_INVAL, "null argument");
+  if (batch > t->p.max_batch) return fail(FECSIM_E_INVAL, "batch above max_batch");
+  TrainArgs a = base_args(t, batch);
+  a.llr = llr;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.post = post;
+  const hipError_t e = launch_train(a, 0, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "train forward launch");
+  t->last_batch = batch;
+  t->alpha = alpha;
+  t->beta = beta;
+  return FECSIM_OK;
+}
+
+int fecsim_train_bwd(fecsim_trainer* t, uint32_t batch, const void* grad_post, void* grad_alpha, void* grad_beta,
+                     void* grad_llr, void* stream) {
+  if (!t || !grad_alpha || !grad_beta || (!grad_post && batch)) return fail(FECSIM_E_INVAL, "null argument");
+  if (batch != t->last_batch || !t->alpha) return fail(FECSIM_E_INVAL, "backward needs the forward of the same batch first");
+  TrainArgs a = base_args(t, batch);
+  a.alpha = t->alpha;
+  a.beta = t->beta;
+  a.grad_post = grad_post;
+  a.grad_alpha = grad_alpha;
+  a.grad_beta = grad_beta;
+  a.grad_llr = grad_llr;
+  const hipError_t e = launch_train(a, 1, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "train backward launch");
+  return FECSIM_OK;
+}
+
+void fecsim_trainer_free(fecsim_trainer* t) {
+  if (!t) return;
+  free_all(t);
+  delete t;
+}
+
+}  // extern "C"

@@ -91,6 +91,7 @@ class Decoder:
                  beta_offset: float = 1.0):
         self.hdec = hdec
         self.kind = kind
+        self.max_iter = max_iter
         p = DecParams(max_iter, max_row_metric, beta_offset)
         d = C.c_void_p()
         _L.check(_L.lib().fecsim_decoder_create(hdec._h, kind.encode(), C.byref(p), C.byref(d)),
@@ -104,6 +105,17 @@ class Decoder:
     def select_kernel(self, kernel: int) -> None:
         """lib.KERNEL_AUTO / KERNEL_GENERIC / KERNEL_PAIR (results are identical)."""
         _L.check(_L.lib().fecsim_decoder_select_kernel(self._d, kernel), "fecsim_decoder_select_kernel")
+
+    def set_offsets(self, beta) -> None:
+        """Per-(iteration, layer) offsets [max_iter, mrow] (converted like beta_offset),
+        or None for the constant beta_offset."""
+        if beta is None:
+            _L.check(_L.lib().fecsim_decoder_set_offsets(self._d, None), "fecsim_decoder_set_offsets")
+            return
+        b = np.ascontiguousarray(beta, dtype=np.float32)
+        if b.shape != (self.max_iter, self.hdec.info["mrow"]):
+            raise ValueError("offsets must have shape (max_iter, mrow) = (%d, %d)" % (self.max_iter, self.hdec.info["mrow"]))
+        _L.check(_L.lib().fecsim_decoder_set_offsets(self._d, b.ctypes.data), "fecsim_decoder_set_offsets")
 
     @property
     def kernel(self) -> int:

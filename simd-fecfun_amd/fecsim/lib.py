@@ -48,6 +48,12 @@ SIGNATURES = [
     ("fecsim_sim_stats_get", _int, [_vp, _vp]),
     ("fecsim_sim_reset", _int, [_vp, _vp]),
     ("fecsim_sim_free", None, [_vp]),
+    ("fecsim_decoder_set_offsets", _int, [_vp, _vp]),
+    ("fecsim_gen_llr_var", _int, [_vp, _vp, C.c_float, _u64, _u64, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),
+    ("fecsim_trainer_create", _int, [_vp, _vp, C.POINTER(_vp)]),
+    ("fecsim_train_fwd", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_train_bwd", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_trainer_free", None, [_vp]),
 ]
 
 

@@ -1,0 +1,282 @@
+"""Training harness for the differentiable layered min-sum decoder.
+
+The decoder itself (forward and hand-written adjoint) runs in the HIP library
+through the C ABI (fecsim_train_fwd / fecsim_train_bwd); this module wraps it in a
+torch.autograd.Function and holds the training loop, the quantization of learned
+offsets into an integer decoder's offset table, and the CWER-vs-SNR comparison that
+measures the gain of the learned offsets.  No counterpart in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import lib as _L
+from .decoder import Decoder, Hdec, _ptr
+
+
+class TrainParams(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("max_batch", C.c_uint32), ("cap", C.c_float), ("elem", C.c_int)]
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Trainer:
+    """Device state of the unrolled decoder: checkpoints for up to max_batch codewords."""
+
+    def __init__(self, hdec: Hdec, iters: int, max_batch: int, cap: float = 1e30, double: bool = False):
+        self.hdec, self.iters, self.max_batch, self.cap = hdec, iters, max_batch, cap
+        self.elem = _L.F64 if double else _L.F32
+        p = TrainParams(iters, max_batch, cap, self.elem)
+        t = C.c_void_p()
+        _L.check(_L.lib().fecsim_trainer_create(hdec._h, C.byref(p), C.byref(t)), "fecsim_trainer_create")
+        self._t = t
+
+    @property
+    def torch_dtype(self):
+        import torch
+        return torch.float64 if self.elem == _L.F64 else torch.float32
+
+    def _check(self, *ts):
+        for t in ts:
+            if t.dtype != self.torch_dtype or not t.is_cuda or not t.is_contiguous():
+                raise TypeError("tensors must be contiguous CUDA %s" % self.torch_dtype)
+
+    def forward(self, llr, alpha, beta):
+        import torch
+        self._check(llr, alpha, beta)
+        post = torch.empty_like(llr)
+        _L.check(_L.lib().fecsim_train_fwd(self._t, llr.shape[0], _ptr(llr), _ptr(alpha), _ptr(beta), _ptr(post),
+                                           _stream()), "fecsim_train_fwd")
+        return post
+
+    def backward(self, grad_post, want_llr=True):
+        import torch
+        grad_post = grad_post.contiguous()
+        self._check(grad_post)
+        ga = torch.empty((self.iters, self.hdec.info["mrow"]), dtype=self.torch_dtype, device="cuda")
+        gb = torch.empty_like(ga)
+        gl = torch.empty_like(grad_post) if want_llr else None
+        _L.check(_L.lib().fecsim_train_bwd(self._t, grad_post.shape[0], _ptr(grad_post), _ptr(ga), _ptr(gb), _ptr(gl),
+                                           _stream()), "fecsim_train_bwd")
+        return gl, ga, gb
+
+    def __del__(self):
+        if getattr(self, "_t", None) and _L._lib is not None:
+            _L.lib().fecsim_trainer_free(self._t)
+            self._t = None
+
+
+def min_sum_function():
+    """torch.autograd.Function: post = decoder(llr; alpha, beta) on the HIP library."""
+    import torch
+
+    class MinSum(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, llr, alpha, beta, trainer):
+            ctx.trainer = trainer
+            ctx.want_llr = llr.requires_grad
+            return trainer.forward(llr.contiguous(), alpha.contiguous(), beta.contiguous())
+
+        @staticmethod
+        def backward(ctx, grad_post):
+            gl, ga, gb = ctx.trainer.backward(grad_post, want_llr=ctx.want_llr)
+            return gl, ga, gb, None
+
+    return MinSum
+
+
+def rate_db(k, n):
+    """10 log10(K/N): Es/N0 = Eb/N0 + rate_db for BPSK."""
+    return 10.0 * math.log10(k / n)
+
+
+def make_batch(henc, batch, ebn0_lo, ebn0_hi, rate, seed, cw0, rng, groups=8, scale=1.0, max_abs=1e9):
+    """Random codewords and float LLRs; the batch is split into `groups` slices, each at
+    an Eb/N0 drawn uniformly from [ebn0_lo, ebn0_hi].  Returns (llr, bits01)."""
+    import torch
+    bits = henc.new_bits(batch)
+    henc.gen_codewords(bits, seed=seed, cw0=cw0)
+    llr = torch.empty((batch, henc.n), dtype=torch.float32, device="cuda")
+    edges = np.linspace(0, batch, groups + 1).astype(int)
+    for g in range(groups):
+        a, b = edges[g], edges[g + 1]
+        if b <= a:
+            continue
+        snr = float(rng.uniform(ebn0_lo, ebn0_hi) + rate)
+        henc.gen_llr(bits[a:b], llr[a:b], _L.F32, snr, seed=seed, cw0=cw0 + a, llr_scale=scale, max_abs_llr=max_abs)
+    z, mcol = henc.z, henc.mcol
+    shifts = torch.arange(32, device="cuda", dtype=torch.int32)
+    b01 = ((bits.unsqueeze(-1) >> shifts) & 1).reshape(batch, mcol, -1)[:, :, :z].reshape(batch, mcol * z)
+    return llr, b01.to(torch.float32)
+
+
+def train(cs, henc, T=10, batch=4096, steps=400, ebn0=(1.0, 3.0), lr=1e-3, init_offset=0.5, init_scale=0.8,
+          learn_scale=True, cap=1e30, seed=7, log_every=50, log=None):
+    """Adam on the BCE of the posteriors (after T iterations) against the transmitted
+    bits.  Returns (alpha [T, mrow], beta [T, mrow], losses)."""
+    import torch
+    import torch.nn.functional as Fn
+    hdec = Hdec.from_tables(cs.dec)
+    tr = Trainer(hdec, T, batch, cap=cap)
+    Lr = cs.dec.mrow
+    alpha = torch.full((T, Lr), init_scale if learn_scale else 1.0, device="cuda", requires_grad=learn_scale)
+    beta = torch.full((T, Lr), init_offset, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([p for p in (alpha, beta) if p.requires_grad], lr=lr)
+    fn = min_sum_function()
+    rng = np.random.default_rng(seed)
+    rate = rate_db(cs.dec.mcol * cs.z - cs.enc.mrow * cs.z, cs.dec.mcol * cs.z)
+    losses = []
+    for step in range(steps):
+        llr, b01 = make_batch(henc, batch, ebn0[0], ebn0[1], rate, seed, step * batch, rng)
+        post = fn.apply(llr, alpha if learn_scale else alpha.detach(), beta, tr)
+        loss = Fn.binary_cross_entropy_with_logits(-post, b01)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            beta.clamp_(min=0.0)
+            if learn_scale:
+                alpha.clamp_(0.05, 1.5)
+        losses.append(float(loss.item()))
+        if log and (step % log_every == 0 or step == steps - 1):
+            log("step %d loss %.5f beta[0] %.3f..%.3f alpha[0] %.3f" % (
+                step, losses[-1], float(beta[0].min()), float(beta[0].max()), float(alpha[0].mean())))
+    return alpha.detach().cpu().numpy(), beta.detach().cpu().numpy(), losses
+
+
+def quantize_offsets(beta, llr_scale=2.0, max_iter=None):
+    """Learned offsets (true LLR units) -> integer offsets of a decoder fed LLRs scaled by
+    llr_scale: round(llr_scale * beta), repeated for iterations past T."""
+    q = np.rint(np.asarray(beta, np.float64) * llr_scale).astype(np.float32)
+    if max_iter is not None and q.shape[0] < max_iter:
+        q = np.concatenate([q, np.repeat(q[-1:], max_iter - q.shape[0], 0)], 0)
+    return q
+
+
+def cwer_curve(cs, henc, dec, snrs, ncw, seed=100, loops=1):
+    """CWER / BER per SNR point of the device simulation loop."""
+    from .sim import Sim
+    sim = Sim(henc, dec, ncw, list(snrs), seed=seed, use_graph=False)
+    sim.run(loops)
+    st = sim.stats()
+    k = (cs.dec.mcol - cs.enc.mrow) * cs.z
+    return [(s, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for s, x in zip(snrs, st)]
+
+
+def snr_at(curve, target=1e-3):
+    """SNR where CWER crosses `target`, log-linear interpolation between the bracketing
+    points; None when the curve does not bracket it."""
+    pts = [(s, c) for s, c, *_ in curve]
+    for (s0, c0), (s1, c1) in zip(pts, pts[1:]):
+        if c0 >= target >= c1 and c0 > 0 and c1 > 0:
+            t = (math.log(c0) - math.log(target)) / (math.log(c0) - math.log(c1))
+            return s0 + t * (s1 - s0)
+    return None
+
+
+def eval_float(cs, henc, alpha, beta, snrs, ncw, T, cap=1e30, batch=4096, seed=1234):
+    """CWER / BER of the unrolled float decoder (T iterations, given alpha/beta) per
+    SNR point (Es/N0 dB), on LLRs of scale 1."""
+    import torch
+    hdec = Hdec.from_tables(cs.dec)
+    tr = Trainer(hdec, T, batch, cap=cap)
+    a = torch.tensor(np.asarray(alpha, np.float32), device="cuda")
+    b = torch.tensor(np.asarray(beta, np.float32), device="cuda")
+    k = (cs.dec.mcol - cs.enc.mrow) * cs.z
+    out = []
+    for snr in snrs:
+        cwe = be = 0
+        done = 0
+        while done < ncw:
+            n = min(batch, ncw - done)
+            bits = henc.new_bits(n)
+            henc.gen_codewords(bits, seed=seed, cw0=done)
+            llr = torch.empty((n, henc.n), dtype=torch.float32, device="cuda")
+            henc.gen_llr(bits, llr, _L.F32, float(snr), seed=seed, cw0=done, llr_scale=1.0, max_abs_llr=1e9)
+            post = tr.forward(llr, a, b)
+            cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+            henc.count_errors(bits, post, _L.F32, cnt)
+            e = cnt.tolist()
+            be += e[0]
+            cwe += e[1]
+            done += n
+        out.append((float(snr), cwe / ncw, be / (ncw * k), float(T)))
+    return out
+
+
+def _main():
+    import argparse
+    import json
+    import sys
+    import time
+    from . import codes
+    from .sim import Henc
+    ap = argparse.ArgumentParser(description="learn per-(iteration, layer) offsets/scales of the layered "
+                                             "min-sum decoder and measure their SNR gain")
+    ap.add_argument("--code", default="11ad 1/2")
+    ap.add_argument("--T", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--ebn0", type=float, nargs=2, default=(1.0, 3.0))
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--init-offset", type=float, default=0.5)
+    ap.add_argument("--init-scale", type=float, default=0.8)
+    ap.add_argument("--offset-only", action="store_true", help="keep the scales at 1 (for the integer decoder)")
+    ap.add_argument("--cap", type=float, default=1e30, help="row cap in LLR units of scale 1")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--eval-snr", type=float, nargs=3, default=None, metavar=("LO", "HI", "STEP"),
+                    help="Es/N0 grid for the CWER comparison")
+    ap.add_argument("--eval-ncw", type=int, default=2_000_000)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    log = lambda s: print(s, file=sys.stderr, flush=True)   # noqa: E731
+    cs = codes.gen_structs(args.code, simd_size=512, elem_size=8)
+    henc = Henc.from_tables(cs.enc)
+    t0 = time.time()
+    alpha, beta, losses = train(cs, henc, T=args.T, batch=args.batch, steps=args.steps, ebn0=tuple(args.ebn0),
+                                lr=args.lr, init_offset=args.init_offset, init_scale=args.init_scale,
+                                learn_scale=not args.offset_only, cap=args.cap, seed=args.seed, log=log,
+                                log_every=max(1, args.steps // 20))
+    res = {"code": args.code, "T": args.T, "batch": args.batch, "steps": args.steps, "ebn0": list(args.ebn0),
+           "lr": args.lr, "init_offset": args.init_offset, "init_scale": args.init_scale,
+           "offset_only": args.offset_only, "cap": args.cap, "seed": args.seed, "train_seconds": time.time() - t0,
+           "loss_first": losses[:5], "loss_last": losses[-5:], "alpha": alpha.tolist(), "beta": beta.tolist()}
+    if args.eval_snr:
+        lo, hi, st = args.eval_snr
+        snrs = [float(x) for x in np.arange(lo, hi + 1e-9, st)]
+        if args.offset_only:
+            # integer decoder (LLR scale 2, |LLR| <= 24, cap 31): plain beta = 1 vs learned table
+            q = quantize_offsets(beta, 2.0, max_iter=args.T)
+            res["beta_int"] = q.tolist()
+            hdec = Hdec.from_tables(cs.dec)
+            plain = Decoder(hdec, "Vec64c_min", args.T, 31.0, 1.0)
+            learned = Decoder(hdec, "Vec64c_min", args.T, 31.0, 1.0)
+            learned.set_offsets(q)
+            cp = cwer_curve(cs, henc, plain, snrs, args.eval_ncw)
+            cl = cwer_curve(cs, henc, learned, snrs, args.eval_ncw)
+            kind = "int8 Vec64c_min, max %d iterations with syndrome exit" % args.T
+        else:
+            # float unrolled decoder: plain offset 0.5 / scale 1 vs learned
+            ones = np.ones_like(alpha)
+            cp = eval_float(cs, henc, ones, 0.5 * ones, snrs, args.eval_ncw, args.T, cap=args.cap)
+            cl = eval_float(cs, henc, alpha, beta, snrs, args.eval_ncw, args.T, cap=args.cap)
+            kind = "float32 unrolled, %d iterations" % args.T
+        sp, sl = snr_at(cp), snr_at(cl)
+        res.update({"eval_decoder": kind, "eval_ncw": args.eval_ncw, "curve_plain": cp, "curve_learned": cl,
+                    "snr_at_1e-3_plain": sp, "snr_at_1e-3_learned": sl,
+                    "gain_db": (sp - sl) if sp is not None and sl is not None else None})
+        log("gain at CWER 1e-3: %s dB" % res["gain_db"])
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps({k: res[k] for k in res if k not in ("alpha", "beta", "beta_int", "curve_plain",
+                                                            "curve_learned")}))
+
+
+if __name__ == "__main__":
+    _main()

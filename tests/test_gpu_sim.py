@@ -241,3 +241,34 @@ def test_gpu_sim_loop_matches_oracle_chain(name, kind, graph, cuda):
     full = _oracle_sim(cs, kind, snrs, seed, num_cw, loops + 1)
     for g, w4, w3 in zip(got, full, want):
         assert g == {k: w4[k] - w3[k] for k in g}
+
+
+def test_gpu_llr_per_codeword_channel_matches_oracle(cuda):
+    """fecsim_gen_llr_var: a (sigma, factor) pair per codeword."""
+    import torch
+    from fecsim import lib as L
+    cs = _cs("11n 1/2 z27")
+    h = Henc.from_tables(cs.enc)
+    ncw, seed, cw0 = 40, 21, 300
+    bits = h.new_bits(ncw)
+    h.gen_codewords(bits, seed=seed, cw0=cw0)
+    dense = refsim.unpack_bits(_words(bits, cs.enc.mcol), cs.z, cs.enc.mcol)
+    rng = np.random.default_rng(3)
+    sf = np.array([chan_sigma(float(s), 2.0) for s in rng.uniform(-2.0, 2.0, ncw)], np.float32)
+    llr = torch.zeros((ncw, cs.enc.mcol * cs.z), dtype=torch.int8, device="cuda")
+    unc = torch.zeros(1, dtype=torch.int64, device="cuda")
+    import ctypes as C
+    from fecsim.decoder import _ptr
+    sft = torch.from_numpy(sf).cuda()
+    L.check(L.lib().fecsim_gen_llr_var(h._h, _ptr(sft), C.c_float(24.0), seed, cw0, ncw, h.col_words, _ptr(bits),
+                                       L.I8, 0, _ptr(llr), _ptr(unc),
+                                       C.c_void_p(torch.cuda.current_stream().cuda_stream)), "gen_llr_var")
+    torch.cuda.synchronize()
+    got = llr.cpu().numpy()
+    tot = 0
+    for i in range(ncw):
+        want, err = O.noise_llr(seed, cw0 + i, dense[i:i + 1], cs.z, cs.enc.mcol, float(sf[i, 0]), float(sf[i, 1]),
+                                24.0, None, None, O.I8)
+        assert np.array_equal(got[i], want[0]), i
+        tot += err
+    assert int(unc.item()) == tot

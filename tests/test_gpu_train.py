@@ -1,0 +1,193 @@
+"""GPU parity of the differentiable decoder and of per-(iteration, layer) offset
+tables: forward bit-exact with the oracle, adjoint against the oracle's FP64 reverse
+mode, FP64 central finite differences and torch autograd."""
+import sys
+
+import numpy as np
+import pytest
+
+from fecsim import codes
+from fecsim.decoder import Decoder, Hdec
+import refsim
+import train_ref
+from refsim import O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _cs(name):
+    return codes.gen_structs(name, simd_size=512, elem_size=8)
+
+
+def _llrs(cs, ncw, snr, elem=O.F32, scale=1.0, mx=1e9, seed=3):
+    rs = refsim.RefStream(cs, seeds=refsim.perl_seeds(seed))
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, snr, elem=elem, llr_scale=scale, max_abs=mx)
+    return llr
+
+
+@pytest.mark.parametrize("kind,snr", [("Vec64c_min", -0.75), ("Vec32s_min", -0.5), ("Vec16f_min", -0.5)])
+def test_gpu_offset_table_matches_oracle(kind, snr, cuda):
+    import torch
+    from fecsim import lib as L
+    cs = _cs("11ad 1/2")
+    elem = O.ELEM_OF_CHAR[kind[kind.index("Vec") + 3:].lstrip("0123456789")[0]]
+    scale, mx = (1.0, 1e9) if elem == O.F32 else (2.0, 24.0)
+    llr = _llrs(cs, 400, snr, elem, scale, mx)
+    rng = np.random.default_rng(2)
+    if elem == O.F32:
+        tab = rng.uniform(0.0, 1.5, (10, cs.dec.mrow)).astype(np.float32)
+    else:
+        tab = rng.integers(0, 4, (10, cs.dec.mrow)).astype(np.float32) + 0.7   # truncates to 0..3
+    want = O.decode_ideal(cs.dec, llr, elem=elem, beta=1.0, beta_table=tab)
+    d = Decoder(Hdec.from_tables(cs.dec), kind, 10, 31.0, 1.0)
+    d.set_offsets(tab)
+    assert d.kernel == L.KERNEL_GENERIC
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    it = torch.zeros(400, dtype=torch.int32, device="cuda")
+    c2v = torch.zeros(400 * d.hdec.e, dtype=x.dtype, device="cuda")
+    d.decode(x, out, iters=it, c2v=c2v)
+    torch.cuda.synchronize()
+    assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
+    assert np.array_equal(out.cpu().numpy().view(np.uint8), want[0].view(np.uint8))
+    assert np.array_equal(c2v.cpu().numpy().reshape(400, -1).view(np.uint8), want[1].view(np.uint8))
+    # a table equal to beta_offset everywhere is the plain decoder; clearing restores the pair kernel
+    d.set_offsets(np.ones((10, cs.dec.mrow), np.float32))
+    out2 = torch.empty_like(x)
+    d.decode(x, out2)
+    d.set_offsets(None)
+    out3 = torch.empty_like(x)
+    d.decode(x, out3)
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out3)
+    if elem == O.I8:
+        assert d.kernel == L.KERNEL_PAIR
+
+
+def _trainer(cs, T, batch, cap, double):
+    from fecsim.train import Trainer
+    return Trainer(Hdec.from_tables(cs.dec), T, batch, cap=cap, double=double)
+
+
+@pytest.mark.parametrize("name,T,cap,double", [("11ad 1/2", 10, 1e30, False), ("11n 1/2 z81", 10, 6.0, False),
+                                               ("11n 1/2 z27", 4, 1e30, True)])
+def test_gpu_train_forward_matches_oracle(name, T, cap, double, cuda):
+    import torch
+    cs = _cs(name)
+    dt = np.float64 if double else np.float32
+    llr = _llrs(cs, 37, -1.0).astype(dt)
+    rng = np.random.default_rng(8)
+    al = rng.uniform(0.6, 1.0, (T, cs.dec.mrow)).astype(dt)
+    be = rng.uniform(0.0, 0.9, (T, cs.dec.mrow)).astype(dt)
+    want, _ = O.train_fwd(cs.dec, llr, al, be, cap=cap, dtype=dt)
+    tr = _trainer(cs, T, 64, cap, double)
+    got = tr.forward(torch.from_numpy(llr).cuda(), torch.from_numpy(al).cuda(), torch.from_numpy(be).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("name,T,double", [("11n 1/2 z27", 5, True), ("11ad 1/2", 4, True), ("11ad 1/2", 4, False),
+                                          ("5GNR_n25344_k8448", 2, True)])
+def test_gpu_train_backward_matches_oracle(name, T, double, cuda):
+    """Adjoint against the oracle's FP64 reverse mode (the BG1 case keeps its
+    posteriors in global memory)."""
+    import torch
+    cs = _cs(name)
+    ncw = 2 if cs.z > 300 else 9
+    x = _llrs(cs, ncw, -0.5 if cs.z < 300 else -4.2).astype(np.float64)
+    rng = np.random.default_rng(9)
+    al = rng.uniform(0.6, 1.0, (T, cs.dec.mrow))
+    be = rng.uniform(0.1, 0.9, (T, cs.dec.mrow))
+    gpost = rng.normal(size=x.shape)
+    cap = 8.0
+    ga, gb, gl = O.train_grad(cs.dec, x, al, be, gpost, cap=cap)
+    dt = torch.float64 if double else torch.float32
+    tr = _trainer(cs, T, ncw, cap, double)
+    t = lambda a: torch.tensor(a, dtype=dt, device="cuda")   # noqa: E731
+    tr.forward(t(x), t(al), t(be))
+    gl2, ga2, gb2 = tr.backward(t(gpost))
+    torch.cuda.synchronize()
+    rtol, atol = (1e-9, 1e-9) if double else (5e-3, 5e-3)
+    sa = max(1.0, np.abs(ga).max())
+    sb = max(1.0, np.abs(gb).max())
+    np.testing.assert_allclose(ga2.cpu().numpy(), ga, rtol=rtol, atol=atol * sa)
+    np.testing.assert_allclose(gb2.cpu().numpy(), gb, rtol=rtol, atol=atol * sb)
+    np.testing.assert_allclose(gl2.cpu().numpy(), gl, rtol=rtol, atol=atol * max(1.0, np.abs(gl).max()))
+    _say("grad", name, T, double, "max|ga|", sa, "max|gb|", sb)
+
+
+def test_gpu_train_gradient_finite_differences_fp64(cuda):
+    """GPU FP64 adjoint against central finite differences (eps 1e-6) of the GPU FP64 forward."""
+    import torch
+    cs = _cs("11n 1/2 z81")
+    T = 10
+    x = torch.tensor(_llrs(cs, 6, -0.5).astype(np.float64), device="cuda")
+    rng = np.random.default_rng(4)
+    al = torch.tensor(rng.uniform(0.6, 1.0, (T, cs.dec.mrow)), device="cuda")
+    be = torch.tensor(rng.uniform(0.1, 0.9, (T, cs.dec.mrow)), device="cuda")
+    w = torch.tensor(rng.normal(size=tuple(x.shape)), device="cuda")
+    tr = _trainer(cs, T, 8, 1e30, True)
+
+    def loss(a, b):
+        return float((tr.forward(x, a, b) * w).sum().item())
+
+    tr.forward(x, al, be)
+    _, ga, gb = tr.backward(w)
+    eps = 1e-6
+    picks = [(int(t), int(l)) for t, l in zip(rng.integers(0, T, 16), rng.integers(0, cs.dec.mrow, 16))]
+    for t_, l_ in picks:
+        for which in ("a", "b"):
+            d = torch.zeros_like(al)
+            d[t_, l_] = eps
+            if which == "a":
+                fd = (loss(al + d, be) - loss(al - d, be)) / (2 * eps)
+                g = float(ga[t_, l_])
+            else:
+                fd = (loss(al, be + d) - loss(al, be - d)) / (2 * eps)
+                g = float(gb[t_, l_])
+            assert abs(fd - g) <= 1e-5 * max(1.0, abs(fd)), (which, t_, l_, fd, g)
+
+
+def test_gpu_autograd_function_matches_torch_reference(cuda):
+    """The torch.autograd.Function over the C ABI against autograd of the independent
+    torch forward (FP64)."""
+    import torch
+    from fecsim.train import min_sum_function
+    cs = _cs("11ad 1/2")
+    T = 6
+    x = _llrs(cs, 5, -0.5).astype(np.float64)
+    rng = np.random.default_rng(6)
+    al = rng.uniform(0.6, 1.0, (T, cs.dec.mrow))
+    be = rng.uniform(0.1, 0.9, (T, cs.dec.mrow))
+    bits = (rng.random(x.shape) < 0.5).astype(np.float64)
+    tr = _trainer(cs, T, 8, 1e30, True)
+    fn = min_sum_function()
+    ta = torch.tensor(al, device="cuda", requires_grad=True)
+    tb = torch.tensor(be, device="cuda", requires_grad=True)
+    tx = torch.tensor(x, device="cuda", requires_grad=True)
+    loss = torch.nn.functional.binary_cross_entropy_with_logits(-fn.apply(tx, ta, tb, tr), torch.tensor(bits, device="cuda"))
+    loss.backward()
+    ra = torch.tensor(al, requires_grad=True)
+    rb = torch.tensor(be, requires_grad=True)
+    rx = torch.tensor(x, requires_grad=True)
+    rl = torch.nn.functional.binary_cross_entropy_with_logits(-train_ref.forward(cs.dec, rx, ra, rb), torch.tensor(bits))
+    rl.backward()
+    assert abs(loss.item() - rl.item()) < 1e-12
+    np.testing.assert_allclose(ta.grad.cpu().numpy(), ra.grad.numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(tb.grad.cpu().numpy(), rb.grad.numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(tx.grad.cpu().numpy(), rx.grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_gpu_training_reduces_loss(cuda):
+    from fecsim.sim import Henc
+    from fecsim.train import train
+    cs = _cs("11ad 1/2")
+    henc = Henc.from_tables(cs.enc)
+    al, be, losses = train(cs, henc, T=5, batch=1024, steps=60, ebn0=(2.0, 3.0), lr=2e-2, log=_say, log_every=20)
+    assert np.mean(losses[-10:]) < np.mean(losses[:10])
+    assert al.shape == (5, cs.dec.mrow) and be.shape == (5, cs.dec.mrow)
