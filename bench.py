@@ -141,9 +141,20 @@ def main():
                     help="short fixed workloads for profiler runs instead of the benchmark: 'traffic' = "
                          "one decode of --ncw codewords at -0.5 dB twice, then a plain copy of the LLR "
                          "tensor twice (counter calibration); 'sim' = --steps simulation loops")
+    ap.add_argument("--config", type=int, choices=[1, 2, 3, 4, 5], default=1,
+                    help="1: 11ad 1/2 int8 decode (the headline metric); 2: 16e 1/2 float sum-product Z sweep; "
+                         "3: mixed-code queue; 4: trainable decoder step; 5: 5G NR BG1 Z=384 int8")
+    ap.add_argument("--z", default=None, help="config 2: comma-separated lifting sizes instead of 24..96")
     args = ap.parse_args()
     if args.probe:
         return probe(args)
+    if args.config != 1:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        from fecsim.benchmarks import run_config
+        print(json.dumps(run_config(args.config, args)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (codewords are "
               "independent, so a batch would shard by codeword range with a final sum of "

@@ -135,6 +135,34 @@ int fecsim_decode_set_host(fecsim_decoder *d, uint32_t num_cw, int layout_p, voi
                            uint32_t *iters_host, unsigned long long *tot_iter);
 
 /* ------------------------------------------------------------------------- */
+/* Mixed-code queue decoder (no reference counterpart)                        */
+/* ------------------------------------------------------------------------- */
+
+/* One persistent launch decodes a queue whose codewords belong to different codes:
+ * decoder c (its tables, max_iter, beta, cap) decodes every queue entry with code id
+ * c, with results identical to fecsim_decode on that decoder.  All decoders need the
+ * same element kind and mode, and their messages must fit one wave's LDS slice. */
+typedef struct fecsim_multi fecsim_multi;
+
+int fecsim_multi_create(fecsim_decoder *const *decs, int ncode, uint32_t max_cw, fecsim_multi **out);
+/* code: device int32 [num_cw] code ids; off: device uint64 [num_cw] element offset of
+ * each codeword's dense LLRs in llr_in (and of its posteriors in post_out); iters:
+ * device [num_cw] or NULL; tot_iter: device uint64 added to, or NULL.  The queue is
+ * grouped by code on the device (no host round trip; graph-capturable). */
+int fecsim_multi_decode(fecsim_multi *m, uint32_t num_cw, const int32_t *code, const unsigned long long *off,
+                        const void *llr_in, void *post_out, uint32_t *iters, unsigned long long *tot_iter,
+                        void *stream);
+/* Autotuner: times the queue (decoded into post_out, best of `rounds`) for candidate
+ * tile shapes -- rows per lane, hence codewords per wave, for each code in turn, then
+ * waves per block -- and keeps the fastest.  Synchronises the stream. */
+int fecsim_multi_autotune(fecsim_multi *m, uint32_t num_cw, const int32_t *code, const unsigned long long *off,
+                          const void *llr_in, void *post_out, int rounds, void *stream);
+int fecsim_multi_shape(const fecsim_multi *m, int code, int *rows_per_lane, int *cw_per_wave,
+                       int *waves_per_block);
+void fecsim_multi_free(fecsim_multi *m);
+
+
+/* ------------------------------------------------------------------------- */
 /* Encoder tables: LDPCencode.raw.dat                                         */
 /* ------------------------------------------------------------------------- */
 
@@ -255,6 +283,32 @@ int fecsim_sim_stats_get(fecsim_sim *s, fecsim_sim_stats *out);
 /* zeroes the counters (stream-ordered); the codeword index keeps advancing */
 int fecsim_sim_reset(fecsim_sim *s, void *stream);
 void fecsim_sim_free(fecsim_sim *s);
+
+/* Mixed-code simulation loop: a fixed queue of num_cw entries, entry i drawn uniformly
+ * from the queue decoder's codes and given an Eb/N0 drawn uniformly from [ebn0_lo,
+ * ebn0_hi] dB (Es/N0 = Eb/N0 + 10 log10(K/N) of its code).  Every loop generates new
+ * codewords and noise for all entries (encoder + channel of each code, LLRs of the
+ * decoders' element type), decodes the interleaved queue with fecsim_multi_decode and
+ * counts errors per code; with use_graph the loop is one replayed HIP graph. */
+typedef struct fecsim_msim fecsim_msim;
+typedef struct {
+    uint32_t num_cw;
+    float ebn0_lo, ebn0_hi;
+    float llr_scale, max_abs_llr;
+    uint64_t seed;
+    int use_graph;
+} fecsim_msim_params;
+/* hencs[c]: encoder tables of the queue decoder's code c */
+int fecsim_msim_create(fecsim_henc *const *hencs, fecsim_multi *m, const fecsim_msim_params *p, fecsim_msim **out);
+int fecsim_msim_run(fecsim_msim *s, uint32_t loops, void *stream);
+/* synchronises; per_code[c] (iters field 0) and the summed iterations of all codes */
+int fecsim_msim_stats(fecsim_msim *s, fecsim_sim_stats *per_code, unsigned long long *iters);
+/* device views of the queue (code ids, LLR offsets), the packed LLR / posterior buffer and
+ * the per-entry (sigma, factor) pairs in code order, for inspection */
+int fecsim_msim_queue(const fecsim_msim *s, const int32_t **code, const unsigned long long **off, void **llr,
+                      const float **sigma_factor);
+int fecsim_msim_reset(fecsim_msim *s, void *stream);
+void fecsim_msim_free(fecsim_msim *s);
 
 /* ------------------------------------------------------------------------- */
 /* Differentiable decoder (no reference counterpart)                          */

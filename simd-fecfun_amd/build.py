@@ -20,7 +20,8 @@ LIB = os.path.join(OUT_DIR, "libfecsim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["decode.hip", "pair_i8.hip", "sim.hip", "train.hip", "api.cpp", "sim_api.cpp", "train_api.cpp"]
+SOURCES = ["decode.hip", "pair_i8.hip", "sim.hip", "train.hip", "multi.hip", "api.cpp", "sim_api.cpp", "msim_api.cpp",
+           "train_api.cpp", "multi_api.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -58,9 +59,36 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_drop_in()
     if verbose:
         print("built", LIB, file=sys.stderr)
     return LIB
+
+
+BIN_DIR = os.path.join(HERE, "bin")
+# executable names of the reference's decoder variants (build/Makefile:9-14), all served
+# by one binary that reads its own name
+DROP_IN_KINDS = ["Vec64c_min", "Vec32s_min", "Vec16w_min", "Vec16f_min", "Vec16f_prod", "Vec8d_min", "Vec8d_prod"]
+
+
+def build_drop_in() -> str:
+    """bin/LDPCtop_fecsim (the reference worker's stdin/stdout protocol on the GPU) and
+    LDPCtop_<kind> links to it for run_LDPCsim.pl."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    exe = os.path.join(BIN_DIR, "LDPCtop_fecsim")
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-x", "hip", os.path.join(CSRC, "ldpctop_main.cpp"),
+           "-o", exe + ".tmp", "-L" + os.path.dirname(LIB), "-lfecsim", "-Wl,-rpath,$ORIGIN/../fecsim"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in executable failed to build:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(exe + ".tmp", exe)
+    for k in DROP_IN_KINDS:
+        link = os.path.join(BIN_DIR, "LDPCtop_" + k)
+        if not os.path.islink(link):
+            if os.path.exists(link):
+                os.remove(link)
+            os.symlink("LDPCtop_fecsim", link)
+    return exe
 
 
 if __name__ == "__main__":

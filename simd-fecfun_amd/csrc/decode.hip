@@ -1,6 +1,7 @@
 // Generic layered decoder kernel: any .dat table, any element kind, dense or the
 // reference's padded memory layout (see decode_core.h for the semantics and the
 // wave/LDS mapping).
+#include <algorithm>
 #include <cstring>
 
 #include "decode_core.h"
@@ -71,45 +72,91 @@ __global__ void __launch_bounds__(256) k_decode(const KernelArgs a) {
 
 static int maxw_bucket(int w) { return w <= 8 ? 8 : w <= 16 ? 16 : w <= 24 ? 24 : w <= 32 ? 32 : 0; }
 
-int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why) {
-  const int Z = h.z;
-  s->maxw = maxw_bucket(h.max_row_weight);
-  if (!s->maxw) { *why = "row weight above 32"; return FECSIM_E_UNSUPPORTED; }
-  const int N = h.mcol * Z, E = h.total_hc_ones * Z;
-  if (N > 65535) { *why = "mcol*z above 65535"; return FECSIM_E_UNSUPPORTED; }
-  // rows per lane: best lane utilization Z*cpw/(64*K); ties -> fewer rows per lane
-  double best = -1.0;
-  for (int K = 1; K <= 16; ++K) {
-    const int L = (Z + K - 1) / K;
-    if (L > 64) continue;
-    const int cpw = 64 / L;
-    const double util = (double)Z * cpw / (64.0 * K);
-    if (util > best + 1e-9) { best = util; s->rows_per_lane = K; s->lanes_per_cw = L; s->cw_per_wave = cpw; }
-  }
-  if (best < 0) { *why = "z above 1024"; return FECSIM_E_UNSUPPORTED; }
+// Shape with K rows per lane: codewords per wave reduced until a wave's slice fits the
+// 64 KB a block may use; C2V in LDS, or in global memory when `global_c2v`.
+static bool fit_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, int K, bool global_c2v, DecShape* s) {
+  const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;
+  const int L = (Z + K - 1) / K;
+  if (L > 64) return false;
   const int pad = 16 / elem_bytes;  // 16-byte aligned slots
+  s->rows_per_lane = K;
+  s->lanes_per_cw = L;
   s->post_stride = (N + pad - 1) / pad * pad;
   s->c2v_stride = (E + pad - 1) / pad * pad;
-  const int lds_cap = 64 * 1024;
-  const int cpw0 = s->cw_per_wave;
-  s->c2v_global = false;
+  s->c2v_global = global_c2v;
+  const int per_slot = (s->post_stride + (global_c2v ? 0 : s->c2v_stride)) * elem_bytes + extra_slot_bytes;
+  s->cw_per_wave = 64 / L;
+  while (s->cw_per_wave > 1 && s->cw_per_wave * per_slot > 65536) --s->cw_per_wave;
+  s->wave_bytes = s->cw_per_wave * per_slot;
+  if (s->wave_bytes > 65536) return false;
+  s->waves = std::min(4, 65536 / s->wave_bytes);
+  return true;
+}
+
+// Lanes doing row work per CU: waves a CU holds within its 160 KB of LDS (at most 32)
+// times the active lanes of a wave.
+static double lanes_per_cu(const DecShape& s) {
+  const int block = s.waves * s.wave_bytes;
+  const int blocks = std::max(1, 160 * 1024 / block);
+  const int waves = std::min(32, blocks * s.waves);
+  return (double)waves * s.cw_per_wave * s.lanes_per_cw;
+}
+
+int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why) {
+  s->maxw = maxw_bucket(h.max_row_weight);
+  if (!s->maxw) { *why = "row weight above 32"; return FECSIM_E_UNSUPPORTED; }
+  if (h.mcol * h.z > 65535) { *why = "mcol*z above 65535"; return FECSIM_E_UNSUPPORTED; }
+  if (h.z > 1024) { *why = "z above 1024"; return FECSIM_E_UNSUPPORTED; }
+  // rows per lane K: most row work in flight per CU once the LDS limits are applied;
+  // ties -> fewer rows per lane.  C2V stays in LDS whenever some K allows it.
+  for (int global = 0; global < 2; ++global) {
+    double best = -1.0;
+    DecShape pick;
+    for (int K = 1; K <= 16; ++K) {
+      DecShape t;
+      t.maxw = s->maxw;
+      if (!fit_shape(h, elem_bytes, extra_slot_bytes, K, global != 0, &t)) continue;
+      const double sc = lanes_per_cu(t);
+      if (sc > best * (1.0 + 1e-9)) {
+        best = sc;
+        pick = t;
+      }
+    }
+    if (best > 0) {
+      *s = pick;
+      return FECSIM_OK;
+    }
+  }
+  *why = "codeword posteriors do not fit one wave's LDS slice";
+  return FECSIM_E_UNSUPPORTED;
+}
+
+int shape_with_rows(const HdecHost& h, int elem_bytes, int K, DecShape* s) {
+  if (K == 0) {
+    std::string why;
+    const int rc = choose_shape(h, elem_bytes, 0, s, &why);
+    return rc || s->c2v_global;
+  }
+  s->maxw = maxw_bucket(h.max_row_weight);
+  const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;
+  const int L = (Z + K - 1) / K;
+  if (!s->maxw || K < 1 || L > 64 || N > 65535) return 1;
+  s->rows_per_lane = K;
+  s->lanes_per_cw = L;
+  s->cw_per_wave = 64 / L;
+  const int pad = 16 / elem_bytes;
+  s->post_stride = (N + pad - 1) / pad * pad;
+  s->c2v_stride = (E + pad - 1) / pad * pad;
   for (;;) {
-    s->wave_bytes = s->cw_per_wave * ((s->post_stride + s->c2v_stride) * elem_bytes + extra_slot_bytes);
-    if (s->wave_bytes <= lds_cap || s->cw_per_wave == 1) break;
+    s->wave_bytes = s->cw_per_wave * (s->post_stride + s->c2v_stride) * elem_bytes;
+    if (s->wave_bytes <= 65536 || s->cw_per_wave == 1) break;
     --s->cw_per_wave;
   }
-  if (s->wave_bytes > lds_cap) {
-    // posteriors in LDS, C2V in a global scratch slot
-    s->c2v_global = true;
-    for (s->cw_per_wave = cpw0;; --s->cw_per_wave) {
-      s->wave_bytes = s->cw_per_wave * (s->post_stride * elem_bytes + extra_slot_bytes);
-      if (s->wave_bytes <= lds_cap || s->cw_per_wave == 1) break;
-    }
-    if (s->wave_bytes > lds_cap) { *why = "codeword posteriors do not fit one wave's LDS slice"; return FECSIM_E_UNSUPPORTED; }
-  }
-  s->waves = lds_cap / s->wave_bytes;
+  if (s->wave_bytes > 65536) return 1;
+  s->waves = 65536 / s->wave_bytes;
   if (s->waves > 4) s->waves = 4;
-  return FECSIM_OK;
+  s->c2v_global = false;
+  return 0;
 }
 
 static long long wrap_to(long long x, int bits) {

@@ -158,6 +158,26 @@ struct TrainArgs {
   void* grad_llr;            // [batch][n] or null
 };
 
+// Mixed-code queue decoder (multi.hip)
+struct MultiArgs {
+  const DecCore* cores;      // [ncode] per-code tables and shapes (C2V in LDS)
+  int ncode;
+  uint32_t num_cw;
+  const int32_t* code;       // [num_cw] code id of every queue entry
+  const unsigned long long* off;  // [num_cw] element offset of its LLRs / posteriors (dense)
+  const void* in;
+  void* out;
+  uint32_t* iters;           // [num_cw] or null
+  unsigned long long* tot_iter;
+  uint32_t* perm;            // [max_cw] queue indices grouped by code
+  uint32_t* cnt;             // [ncode] histogram (zero between calls)
+  uint32_t* cursor;          // [ncode]
+  uint32_t* code_start;      // [ncode + 1]
+  uint32_t* item_start;      // [ncode + 1]
+  uint32_t* counter;         // next work item
+  int wave_bytes, waves;     // LDS per wave (max over codes), waves per block
+};
+
 // Layout of the row-pair int8 kernel (pair_i8.hip)
 struct PairLayout {
   int half = 0, cw_per_wave = 0, col_dwords = 0, slot_dwords = 0, waves = 0;
@@ -191,6 +211,12 @@ int generic_resident_blocks(const DecoderDev& d);
 // device form of a per-(iteration, layer) offset table (long long for integer kinds,
 // the bits of a double for float kinds)
 void offsets_table(const DecoderDev& d, const float* beta, size_t n, std::vector<long long>* out);
+// mixed-code decoder: blocks resident for its kernel, and the launch (sort + decode)
+int multi_resident_blocks(int elem, int mode, int maxw, int waves, size_t lds);
+hipError_t launch_multi(int elem, int mode, int maxw, const MultiArgs& a, uint32_t grid, hipStream_t stream);
+// a DecShape with `rows_per_lane` rows per lane (0: the best lane utilisation), C2V in LDS;
+// returns non-zero when it does not fit
+int shape_with_rows(const HdecHost& h, int elem_bytes, int rows_per_lane, DecShape* s);
 // forward (bwd = 0) or adjoint + parameter-gradient sum (bwd = 1) of the trainable decoder
 hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t stream);
 int encode_shape(int z, int mcol, EncArgs* a);
@@ -231,3 +257,38 @@ struct fecsim_hdec {
   fecsim::HdecHost h;
   int32_t len_f2b = 0, len_b2f = 0;
 };
+
+struct fecsim_henc {
+  fecsim::HencHost h;
+  int device = -1;  // device holding the tables below (-1: not uploaded yet)
+  int32_t* d_cnt = nullptr;
+  int32_t* d_colnum = nullptr;
+  uint32_t* d_src = nullptr;
+  uint32_t* d_punct = nullptr;
+  uint32_t* d_fill = nullptr;
+};
+
+struct fecsim_multi {
+  std::vector<fecsim_decoder*> decs;
+  int elem = 0, mode = 0, maxw = 8;
+  std::vector<fecsim::DecShape> shapes;  // tile shape per code
+  int waves_pref = 4;                     // waves per block asked for
+  int waves = 4, wave_bytes = 0;          // in use
+  uint32_t grid = 0, max_cw = 0;
+  fecsim::DecCore* d_cores = nullptr;
+  uint32_t* d_ws = nullptr;               // perm | cnt | cursor | code_start | item_start | counter
+};
+
+namespace fecsim {
+// encoder / channel / count launch helpers shared by the simulation loops (sim_api.cpp)
+int ensure_device(fecsim_henc* h);
+int enc_args(fecsim_henc* h, int gen, uint64_t seed, uint64_t cw0, const unsigned long long* cw0_dev,
+             uint32_t num_cw, int col_words, uint32_t* bits, EncArgs* a);
+void chan_sigma(float snr, float llr_scale, float* sigma, float* factor);
+int llr_args(fecsim_henc* h, float sigma, float factor, float max_abs, uint64_t seed, uint64_t cw0,
+             const unsigned long long* cw0_dev, uint32_t num_cw, int col_words, const uint32_t* bits, int layout_p,
+             void* llr, unsigned long long* uncoded, LlrArgs* a);
+void count_args(const fecsim_henc* h, uint32_t num_cw, int col_words, const uint32_t* bits, int layout_p,
+                const void* post, unsigned long long* counts, uint32_t* cw_bit_err, unsigned long long* cw_counter,
+                CountArgs* a);
+}  // namespace fecsim

@@ -158,6 +158,10 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
       const short2v mv1 = smin(cap2, m1 - beta2);
       const short2v mv2 = smin(cap2, m2 - beta2);
       const short2v sumb = m1 + m2 - beta2;
+      // all of the layer's results first, then the stores: the per-edge chains are
+      // independent, so the scheduler can interleave them (no back-to-back dependent
+      // packed ops waiting on each other)
+      uint32_t gu[MAXW];
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
@@ -167,9 +171,14 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           short2v g = q[j] + as_s(rn);
           g = smin(smax(g, lo8), hi8);
           c2v[e0 + j] = rn;
-          const uint32_t gu = as_u(g);
-          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] & 0xffffu)) = gu;
-          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] >> 16)) = __builtin_amdgcn_alignbit(gu, gu, 16);
+          gu[j] = as_u(g);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] & 0xffffu)) = gu[j];
+          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] >> 16)) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);
         }
       }
       e0 += w;

@@ -128,16 +128,17 @@ __global__ void __launch_bounds__(256) k_encode(EncArgs a) {
 // ---------------------------------------------------------------------------
 // noise + quantized LLRs: one thread per (codeword, column pair, row)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void noise_pair(uint64_t seed, uint64_t p, float* s_out, float* c_out) {
+// hp = (p + 1) * GAMMA for pair counter p
+__device__ __forceinline__ void noise_pair(uint64_t seed, uint64_t hp, float* s_out, float* c_out) {
   // u = float((r & 0x7fffffff) ^ 0x7fffffff) * 2^-31, an extra draw scaled by 2^-62
   // (plus 2^-63) when the top 28 bits are zero, g = sqrt(-2 ln u), angle = int32 * 2^-31
   // (prng_functions.cpp:148-186, one pair per counter)
-  const uint64_t r = fecmath::splitmix64(seed + (p + 1) * kGamma);
+  const uint64_t r = fecmath::splitmix64(seed + hp);
   const int32_t lo = (int32_t)(uint32_t)r, hi = (int32_t)(r >> 32);
   const int32_t v = (lo & 0x7fffffff) ^ 0x7fffffff;
   float u = (float)v * 0x1.0p-31f;
   if ((v & 0x7ffffff8) == 0) {
-    const uint64_t r2 = fecmath::splitmix64((seed ^ kExtraKey) + (p + 1) * kGamma);
+    const uint64_t r2 = fecmath::splitmix64((seed ^ kExtraKey) + hp);
     const int32_t v2 = ((int32_t)(uint32_t)r2 & 0x7fffffff) ^ 0x7fffffff;
     u = u + __builtin_fmaf((float)v2, 0x1.0p-62f, 0x1.0p-63f);
   }
@@ -180,15 +181,24 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
   const unsigned long long cw_base = a.cw0 + (a.cw0_dev ? *a.cw0_dev : 0ull);
   T* out = reinterpret_cast<T*>(a.out);
   unsigned long long err = 0;
-  const uint32_t total = a.num_cw * per_cw;   // host splits launches below 2^31
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const uint32_t cw = fdiv(t, a.div_cw);
-    const uint32_t rem = t - cw * per_cw;
-    const uint32_t k = fdiv(rem, a.div_z);
-    const int r = (int)(rem - k * (uint32_t)a.z);
-    const uint64_t p = ((uint64_t)(cw_base + a.cw_off + cw) * (uint64_t)a.hm + k) * (uint64_t)a.z + (uint64_t)r;
+  const uint32_t total = a.num_cw * per_cw;   // host splits launches below 2^30
+  // The pair counter is linear in the flat index t = (cw*hm + k)*z + r: p = p0 + t.  The
+  // loop advances (cw, k, r) and (p + 1) * GAMMA by the grid stride instead of dividing
+  // and multiplying for every item.
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cw = fdiv(t, a.div_cw);
+  uint32_t k = fdiv(t - cw * per_cw, a.div_z);
+  uint32_t r = t - cw * per_cw - k * (uint32_t)a.z;
+  const uint32_t dcw = fdiv(stride, a.div_cw);
+  const uint32_t dk = fdiv(stride - dcw * per_cw, a.div_z);
+  const uint32_t dr = stride - dcw * per_cw - dk * (uint32_t)a.z;
+  const uint64_t p0 = (uint64_t)(cw_base + a.cw_off) * per_cw;
+  uint64_t hp = (p0 + 1 + t) * kGamma;
+  const uint64_t dhp = (uint64_t)stride * kGamma;
+  for (; t < total; t += stride) {
     float ns, nc;
-    noise_pair(a.seed, p, &ns, &nc);
+    noise_pair(a.seed, hp, &ns, &nc);
     const uint64_t gcw = a.cw_off + cw;
     float sigma = a.sigma, factor = a.factor;
     if (a.sf) {
@@ -214,6 +224,18 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
       }
       out[(long long)gcw * a.cw_stride + (long long)col * a.col_stride + a.base + r] = to_elem<T>(l);
     }
+    hp += dhp;
+    r += dr;
+    k += dk;
+    cw += dcw;
+    if (r >= (uint32_t)a.z) {
+      r -= a.z;
+      ++k;
+    }
+    if (k >= (uint32_t)a.hm) {
+      k -= a.hm;
+      ++cw;
+    }
   }
   const unsigned long long s = block_sum(err, red);
   if (threadIdx.x == 0 && s && a.uncoded) atomicAdd(a.uncoded, s);
@@ -235,21 +257,37 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   __syncthreads();
   unsigned long long bit_err = 0, cw_err = 0;
   const uint64_t ntiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
+  // item f of a tile = (codeword lc, data column c, row r), advanced by blockDim per step
+  const uint32_t dlc = fdiv(blockDim.x, a.div_dz);
+  const uint32_t dc = fdiv(blockDim.x - dlc * dz, a.div_z);
+  const uint32_t dr = blockDim.x - dlc * dz - dc * (uint32_t)a.z;
+  const uint32_t lc0 = fdiv(threadIdx.x, a.div_dz);
+  const uint32_t c0 = fdiv(threadIdx.x - lc0 * dz, a.div_z);
+  const uint32_t r0 = threadIdx.x - lc0 * dz - c0 * (uint32_t)a.z;
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t cw_first = tile * a.cpb;
+    uint32_t lc = lc0, c = c0, r = r0;
     for (uint32_t f = threadIdx.x; f < items; f += blockDim.x) {
-      const uint32_t lc = fdiv(f, a.div_dz);
-      const uint32_t rem = f - lc * dz;
-      const uint32_t c = fdiv(rem, a.div_z);
-      const int r = (int)(rem - c * (uint32_t)a.z);
       const uint64_t cw = cw_first + lc;
-      if (cw >= a.num_cw) continue;
-      const T v = post[(long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base + r];
-      const uint32_t hard = v < T(0) ? 1u : 0u;
-      const uint32_t bit = (a.bits[(cw * a.mcol + c) * a.col_words + (r >> 5)] >> (r & 31)) & 1u;
-      if (hard != bit) {
-        atomicAdd(&s_cnt[lc], 1u);
-        atomicOr(&s_mask[lc], 1ull << (r % a.num_par));
+      if (cw < a.num_cw) {
+        const T v = post[(long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base + r];
+        const uint32_t hard = v < T(0) ? 1u : 0u;
+        const uint32_t bit = (a.bits[(cw * a.mcol + c) * a.col_words + (r >> 5)] >> (r & 31)) & 1u;
+        if (hard != bit) {
+          atomicAdd(&s_cnt[lc], 1u);
+          atomicOr(&s_mask[lc], 1ull << (r % a.num_par));
+        }
+      }
+      r += dr;
+      c += dc;
+      lc += dlc;
+      if (r >= (uint32_t)a.z) {
+        r -= a.z;
+        ++c;
+      }
+      if (c >= (uint32_t)a.ndata) {
+        c -= a.ndata;
+        ++lc;
       }
     }
     __syncthreads();

@@ -13,16 +13,6 @@
 
 using namespace fecsim;
 
-struct fecsim_henc {
-  HencHost h;
-  int device = -1;  // device holding the tables below (-1: not uploaded yet)
-  int32_t* d_cnt = nullptr;
-  int32_t* d_colnum = nullptr;
-  uint32_t* d_src = nullptr;
-  uint32_t* d_punct = nullptr;
-  uint32_t* d_fill = nullptr;
-};
-
 struct fecsim_sim {
   fecsim_henc* h = nullptr;
   fecsim_decoder* d = nullptr;
@@ -136,6 +126,10 @@ void free_tables(fecsim_henc* h) {
   h->device = -1;
 }
 
+}  // namespace
+
+namespace fecsim {
+
 int ensure_device(fecsim_henc* h) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -232,6 +226,10 @@ void count_args(const fecsim_henc* h, uint32_t num_cw, int col_words, const uint
   a->cw_bit_err = cw_bit_err;
   a->cw_counter = cw_counter;
 }
+
+}  // namespace fecsim
+
+namespace {
 
 bool elem_ok(int elem) { return elem >= FECSIM_I8 && elem <= FECSIM_F64; }
 

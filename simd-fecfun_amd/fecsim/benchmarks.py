@@ -1,0 +1,221 @@
+"""Measurements of the workloads beside the headline one (bench.py --config N).
+
+  2  802.16e rate 1/2, Z = 24..96 step 4, float32 layered sum-product (Vec16f_prod:
+     scale 1.0, |LLR| <= 1e9, beta 0, max_row_metric 6e-8, run_LDPCsim.pl:56-63), max 10
+     iterations, Eb/N0 0.5..2.5 dB, 2^18 codewords per (Z, Eb/N0) point
+  3  mixed-code queue: 65536 codewords per loop, code uniform over 11n 1/2 z27,
+     11n 3/4 z54, 11n 5/6 z81, 11ad 13/16, 16e 2/3A Z=96, Eb/N0 uniform in [1, 4] dB,
+     int8 offset min-sum, per-code max iterations 8/12/16/20/10, autotuned tile shapes,
+     one HIP graph per loop (encode + WGN/LLR + decode + count)
+  4  trainable decoder: 11n 1/2 z81, T = 10, batch 4096, forward + adjoint per step
+  5  5G NR base graph 1, Z = 384, rate 1/3 (2Z punctured), int8 offset min-sum, max 20
+     iterations, 16384-codeword batches, Eb/N0 0..1.5 dB; throughput at CWER 1e-3
+
+Every function returns one JSON-serialisable dict.  Data are synthetic (the library's
+counter-based codewords and noise).
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+
+import numpy as np
+
+
+def _log(s):
+    print(s, file=sys.stderr, flush=True)
+
+
+def _rate_db(cs):
+    return 10 * math.log10((cs.enc.mcol - cs.enc.mrow) / cs.enc.mcol)
+
+
+def config2(args):
+    import torch
+    from . import codes
+    from .decoder import Decoder, Hdec
+    from .sim import Henc, Sim
+    ebn0 = [0.5, 1.0, 1.5, 2.0, 2.5]
+    ncw = 1 << 18
+    per_z, tot_cw, tot_t = [], 0, 0.0
+    zs = [int(x) for x in args.z.split(",")] if getattr(args, "z", None) else list(range(24, 97, 4))
+    for z in zs:
+        cs = codes.gen_structs("16e 1/2", simd_size=512, elem_size=32, z_over=z)
+        henc = Henc.from_tables(cs.enc)
+        dec = Decoder(Hdec.from_tables(cs.dec), "Vec16f_prod", 10, 6e-8, 0.0)
+        snrs = [e + _rate_db(cs) for e in ebn0]
+        sim = Sim(henc, dec, ncw, snrs, llr_scale=1.0, max_abs_llr=1e9, seed=100 + z, use_graph=True)
+        sim.run(1)
+        torch.cuda.synchronize()
+        sim.reset()
+        t0 = time.perf_counter()
+        sim.run(1)
+        st = sim.stats()
+        dt = time.perf_counter() - t0
+        k = (cs.enc.mcol - cs.enc.mrow) * cs.z
+        per_z.append({"z": z, "seconds": dt, "cw_per_s": ncw * len(snrs) / dt,
+                      "ber": [x["bit_err"] / (x["cw"] * k) for x in st],
+                      "cwer": [x["cw_err"] / x["cw"] for x in st],
+                      "mean_iterations": [x["iters"] / x["cw"] for x in st]})
+        tot_cw += ncw * len(snrs)
+        tot_t += dt
+        _log("config 2: z=%d %.3f s cwer %s" % (z, dt, ["%.2e" % c for c in per_z[-1]["cwer"]]))
+    return {"config": 2, "workload": "16e 1/2 Z=24..96 step 4, Vec16f_prod (float32 sum-product), max 10 iterations, "
+                                     "Eb/N0 0.5..2.5 dB step 0.5, 2^18 cw per point, simulation loop per Z",
+            "ebn0_db": ebn0, "cw_per_s": tot_cw / tot_t, "codewords": tot_cw, "seconds": tot_t, "per_z": per_z,
+            "dtype": "f32", "data": "synthetic"}
+
+
+CODES3 = [("11n 1/2 z27", None, 8), ("11n 3/4 z54", None, 12), ("11n 5/6 z81", None, 16),
+          ("11ad 13/16", None, 20), ("16e 2/3A", 96, 10)]
+
+
+def config3(args):
+    import torch
+    from . import codes
+    from .decoder import Decoder, Hdec
+    from .multi import MixedSim, MultiDecoder
+    from .sim import Henc
+    n = 65536
+    decs, hencs, css = [], [], []
+    for name, zo, mi in CODES3:
+        cs = codes.gen_structs(name, simd_size=512, elem_size=8, z_over=zo)
+        css.append(cs)
+        decs.append(Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, 31.0, 1.0))
+        hencs.append(Henc.from_tables(cs.enc))
+    m = MultiDecoder(decs, n)
+    shapes0 = [m.shape(c) for c in range(len(CODES3))]
+    # autotune on one loop's queue (its LLRs copied aside, decoded repeatedly)
+    tune = MixedSim(hencs, m, n, ebn0=(1.0, 4.0), seed=5, use_graph=False)
+    tune.run(1)
+    torch.cuda.synchronize()
+    import ctypes as C
+    from . import lib as _L
+    code, off, llr, sf = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _L.check(_L.lib().fecsim_msim_queue(tune._s, C.byref(code), C.byref(off), C.byref(llr), C.byref(sf)), "queue")
+    nbytes = int(max(cs.enc.mcol * cs.z for cs in css) * n)   # bound of the packed LLR buffer
+    scratch = torch.empty(nbytes, dtype=torch.int8, device="cuda")
+    _L.check(_L.lib().fecsim_multi_autotune(m._m, n, code, off, llr, C.c_void_p(scratch.data_ptr()), 3,
+                                            C.c_void_p(torch.cuda.current_stream().cuda_stream)), "autotune")
+    shapes1 = [m.shape(c) for c in range(len(CODES3))]
+    del tune
+    sim = MixedSim(hencs, m, n, ebn0=(1.0, 4.0), seed=100, use_graph=True)
+    sim.run(1)
+    torch.cuda.synchronize()
+    sim.reset()
+    loops = max(1, args.steps)
+    t0 = time.perf_counter()
+    sim.run(loops)
+    st, iters = sim.stats()
+    dt = time.perf_counter() - t0
+    info_bits = sum(s["cw"] * (cs.enc.mcol - cs.enc.mrow) * cs.z for s, cs in zip(st, css))
+    return {"config": 3, "workload": "mixed queue of 65536 cw per loop over 5 codes (11n 1/2 z27, 11n 3/4 z54, "
+                                     "11n 5/6 z81, 11ad 13/16, 16e 2/3A Z=96), Eb/N0 U[1,4] dB, int8 min-sum, max "
+                                     "iterations 8/12/16/20/10, HIP graph per loop (encode+WGN+decode+count)",
+            "loops": loops, "seconds": dt, "cw_per_s": n * loops / dt, "info_gbit_s": info_bits / dt / 1e9,
+            "mean_iterations": iters / (n * loops),
+            "tile_shapes_default": shapes0, "tile_shapes_autotuned": shapes1,
+            "per_code": [{"code": c[0], "cw": s["cw"], "cwer": s["cw_err"] / max(1, s["cw"])}
+                         for c, s in zip(CODES3, st)],
+            "dtype": "int8", "data": "synthetic"}
+
+
+def config4(args):
+    import torch
+    from . import codes
+    from .decoder import Hdec
+    from .sim import Henc
+    from .train import Trainer, make_batch, min_sum_function, rate_db
+    cs = codes.gen_structs("11n 1/2 z81", simd_size=512, elem_size=8)
+    henc = Henc.from_tables(cs.enc)
+    T, B = 10, 4096
+    tr = Trainer(Hdec.from_tables(cs.dec), T, B)
+    alpha = torch.full((T, cs.dec.mrow), 0.8, device="cuda", requires_grad=True)
+    beta = torch.full((T, cs.dec.mrow), 0.5, device="cuda", requires_grad=True)
+    fn = min_sum_function()
+    rng = np.random.default_rng(1)
+    rate = rate_db(cs.dec.mcol * cs.z - cs.enc.mrow * cs.z, cs.dec.mcol * cs.z)
+    llr, b01 = make_batch(henc, B, 1.0, 3.0, rate, 7, 0, rng)
+    steps = max(3, args.steps)
+    for _ in range(2):
+        post = fn.apply(llr, alpha, beta, tr)
+        torch.nn.functional.binary_cross_entropy_with_logits(-post, b01).backward()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd_ms = bwd_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev[0].record()
+        post = fn.apply(llr, alpha, beta, tr)
+        loss = torch.nn.functional.binary_cross_entropy_with_logits(-post, b01)
+        ev[1].record()
+        loss.backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd_ms += ev[0].elapsed_time(ev[1])
+        bwd_ms += ev[1].elapsed_time(ev[2])
+    dt = time.perf_counter() - t0
+    E = cs.dec.total_hc_ones * cs.z
+    return {"config": 4, "workload": "11n 1/2 z81 trainable offset/normalized min-sum, T=10, batch 4096, "
+                                     "forward + adjoint (checkpoint per iteration, recomputation)",
+            "steps": steps, "step_ms": dt / steps * 1e3, "forward_ms": fwd_ms / steps, "backward_ms": bwd_ms / steps,
+            "cw_per_s_train": B * steps / dt, "edge_updates_per_s_forward": B * T * E / (fwd_ms / steps * 1e-3),
+            "dtype": "f32", "data": "synthetic"}
+
+
+def config5(args):
+    import torch
+    from . import codes
+    from .decoder import Decoder, Hdec
+    from .sim import Henc, Sim
+    from .train import cwer_curve, snr_at
+    cs = codes.gen_structs("5GNR_n25344_k8448", simd_size=512, elem_size=8)
+    henc = Henc.from_tables(cs.enc)
+    dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
+    rate = 10 * math.log10(8448 / 25344)          # transmitted rate 1/3 (2Z columns punctured)
+    ebn0 = [0.0, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5]
+    snrs = [e + rate for e in ebn0]
+    ncw = 16384
+    loops = max(1, args.steps)
+    sim = Sim(henc, dec, ncw, snrs, seed=100, use_graph=True)
+    sim.run(1)
+    torch.cuda.synchronize()
+    sim.reset()
+    t0 = time.perf_counter()
+    sim.run(loops)
+    st = sim.stats()
+    dt = time.perf_counter() - t0
+    k = 8448
+    curve = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for e, x in zip(ebn0, st)]
+    e3 = snr_at(curve)
+    # decode-only throughput at the Eb/N0 closest to CWER 1e-3
+    pick = min(range(len(ebn0)), key=lambda i: abs(math.log10(max(curve[i][1], 1e-9)) + 3))
+    bits = henc.new_bits(ncw)
+    henc.gen_codewords(bits, seed=7)
+    from . import lib as _L
+    llr = torch.empty((ncw, henc.n), dtype=torch.int8, device="cuda")
+    henc.gen_llr(bits, llr, _L.I8, snrs[pick], seed=7)
+    out = torch.empty_like(llr)
+    it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    dec.decode(llr, out, iters=it)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        dec.decode(llr, out, iters=it)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"config": 5, "workload": "5G NR BG1 Z=384 (K=8448, N=25344 sent, 2Z punctured), int8 offset min-sum "
+                                     "beta 1 cap 31, max 20 iterations, 16384 cw per batch, Eb/N0 0..1.5 dB",
+            "sim_seconds": dt, "sim_cw_per_s": ncw * len(snrs) * loops / dt,
+            "curve_ebn0_cwer_ber_iters": curve, "ebn0_at_cwer_1e-3": e3,
+            "decode_at_ebn0": ebn0[pick], "decode_cw_per_s": ncw / (ms * 1e-3),
+            "decode_info_gbit_s": ncw * k / (ms * 1e-3) / 1e9, "decode_mean_iterations": float(it.float().mean()),
+            "dtype": "int8", "data": "synthetic"}
+
+
+def run_config(n, args):
+    return {2: config2, 3: config3, 4: config4, 5: config5}[n](args)

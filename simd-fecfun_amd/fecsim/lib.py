@@ -49,6 +49,17 @@ SIGNATURES = [
     ("fecsim_sim_reset", _int, [_vp, _vp]),
     ("fecsim_sim_free", None, [_vp]),
     ("fecsim_decoder_set_offsets", _int, [_vp, _vp]),
+    ("fecsim_multi_create", _int, [_vp, _int, _u32, C.POINTER(_vp)]),
+    ("fecsim_multi_decode", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_multi_autotune", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _int, _vp]),
+    ("fecsim_multi_shape", _int, [_vp, _int, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
+    ("fecsim_multi_free", None, [_vp]),
+    ("fecsim_msim_create", _int, [_vp, _vp, _vp, C.POINTER(_vp)]),
+    ("fecsim_msim_run", _int, [_vp, _u32, _vp]),
+    ("fecsim_msim_stats", _int, [_vp, _vp, C.POINTER(C.c_ulonglong)]),
+    ("fecsim_msim_queue", _int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    ("fecsim_msim_reset", _int, [_vp, _vp]),
+    ("fecsim_msim_free", None, [_vp]),
     ("fecsim_gen_llr_var", _int, [_vp, _vp, C.c_float, _u64, _u64, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),
     ("fecsim_trainer_create", _int, [_vp, _vp, C.POINTER(_vp)]),
     ("fecsim_train_fwd", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
@@ -76,6 +87,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libfecsim.so not built at %s (run simd-fecfun_amd/build.py)" % LIB_PATH)
+        # torch ships its own HIP runtime with the same soname (libamdhip64.so.7) as the
+        # system one libfecsim.so links.  Loading torch first makes libfecsim.so bind to
+        # that runtime, so one runtime owns the device, the streams and the memory that
+        # both use; loaded the other way round torch would open a second runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)

@@ -159,6 +159,53 @@ def quantize_offsets(beta, llr_scale=2.0, max_iter=None):
     return q
 
 
+def refine_offsets(cs, henc, table, snr, ncw=2_000_000, sweeps=2, seed=4242, max_iter=10, log=None):
+    """Greedy +-1 coordinate descent on an integer offset table of the int8 decoder
+    (LLR scale 2, |LLR| <= 24, cap 31): every candidate is scored by its codeword
+    errors on the same pre-generated batch at `snr` (paired comparison), a change is
+    kept when it removes errors.  Returns (table, errors before, errors after)."""
+    import torch
+    hdec = Hdec.from_tables(cs.dec)
+    dec = Decoder(hdec, "Vec64c_min", max_iter, 31.0, 1.0)
+    bits = henc.new_bits(ncw)
+    henc.gen_codewords(bits, seed=seed, cw0=0)
+    llr = torch.empty((ncw, henc.n), dtype=torch.int8, device="cuda")
+    henc.gen_llr(bits, llr, _L.I8, float(snr), seed=seed, llr_scale=2.0, max_abs_llr=24.0)
+    post = torch.empty_like(llr)
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+    def errors(tab):
+        dec.set_offsets(tab)
+        dec.decode(llr, post)
+        cnt.zero_()
+        henc.count_errors(bits, post, _L.I8, cnt)
+        return int(cnt[1].item())
+
+    tab = np.array(table, np.float32).copy()
+    first = best = errors(tab)
+    for sw in range(sweeps):
+        changed = 0
+        for t in range(tab.shape[0]):
+            for l in range(tab.shape[1]):
+                for d in (-1.0, 1.0):
+                    v = tab[t, l] + d
+                    if v < 0 or v > 6:
+                        continue
+                    old = tab[t, l]
+                    tab[t, l] = v
+                    e = errors(tab)
+                    if e < best:
+                        best = e
+                        changed += 1
+                        break
+                    tab[t, l] = old
+        if log:
+            log("refine sweep %d: %d codeword errors (start %d), %d entries changed" % (sw, best, first, changed))
+        if not changed:
+            break
+    return tab, first, best
+
+
 def cwer_curve(cs, henc, dec, snrs, ncw, seed=100, loops=1):
     """CWER / BER per SNR point of the device simulation loop."""
     from .sim import Sim
@@ -233,6 +280,9 @@ def _main():
     ap.add_argument("--eval-snr", type=float, nargs=3, default=None, metavar=("LO", "HI", "STEP"),
                     help="Es/N0 grid for the CWER comparison")
     ap.add_argument("--eval-ncw", type=int, default=2_000_000)
+    ap.add_argument("--refine-snr", type=float, default=None,
+                    help="Es/N0 at which the quantized offset table is refined by coordinate descent")
+    ap.add_argument("--refine-sweeps", type=int, default=2)
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     log = lambda s: print(s, file=sys.stderr, flush=True)   # noqa: E731
@@ -253,6 +303,11 @@ def _main():
         if args.offset_only:
             # integer decoder (LLR scale 2, |LLR| <= 24, cap 31): plain beta = 1 vs learned table
             q = quantize_offsets(beta, 2.0, max_iter=args.T)
+            res["beta_int_rounded"] = q.tolist()
+            if args.refine_snr is not None:
+                q, e0, e1 = refine_offsets(cs, henc, q, args.refine_snr, sweeps=args.refine_sweeps, max_iter=args.T,
+                                           log=log)
+                res.update({"refine_snr": args.refine_snr, "refine_errors_before": e0, "refine_errors_after": e1})
             res["beta_int"] = q.tolist()
             hdec = Hdec.from_tables(cs.dec)
             plain = Decoder(hdec, "Vec64c_min", args.T, 31.0, 1.0)
