@@ -206,15 +206,20 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
     fecsim_decoder_free(dec);
     return hip_fail(he, "decoder tables to device");
   }
+  he = prepare_generic(d);
+  if (he != hipSuccess) {
+    fecsim_decoder_free(dec);
+    return hip_fail(he, "decoder LDS limit");
+  }
   if (d.shape.c2v_global) {
-    // one C2V slot per resident (block, wave, codeword slot); the kernel strides over the batch
+    // one C2V slot per resident workgroup; the kernel strides over the batch
     const int blocks = generic_resident_blocks(d);
     if (blocks < 1) {
       fecsim_decoder_free(dec);
       return fail(FECSIM_E_HIP, "occupancy query failed");
     }
     d.max_grid = (uint32_t)blocks;
-    const size_t bytes = (size_t)blocks * d.shape.waves * d.shape.cw_per_wave * d.shape.c2v_stride * d.elem_bytes;
+    const size_t bytes = (size_t)blocks * d.shape.c2v_stride * d.elem_bytes;
     he = hipMalloc(&d.d_c2v_scratch, bytes);
     if (he != hipSuccess) {
       fecsim_decoder_free(dec);

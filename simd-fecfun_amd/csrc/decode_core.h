@@ -71,6 +71,65 @@ __device__ __forceinline__ bool is_neg(typename P::S x) {
 }
 
 // ---------------------------------------------------------------------------
+// codeword transfer between global memory and LDS
+// ---------------------------------------------------------------------------
+
+// Rows r0, r0 + step, ... of every column, eight columns per batch so eight loads
+// are in flight before the first store waits (the last batch re-reads the last
+// column instead of branching).  `gs` is the column stride in global memory; LDS
+// columns are packed (stride Z).
+template <class S>
+__device__ __forceinline__ void cols_to_lds(S* __restrict__ post, const S* __restrict__ g, long long base,
+                                            long long gs, int mcol, int Z, int r0, int step) {
+  for (int r = r0; r < Z; r += step)
+    for (int c0 = 0; c0 < mcol; c0 += 8) {
+      S t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = g[base + min(c0 + u, mcol - 1) * gs + r];   // unconditional: one wait
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < mcol) post[(c0 + u) * Z + r] = t[u];
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void cols_from_lds(S* __restrict__ g, const S* __restrict__ post, long long base,
+                                              long long gs, int mcol, int Z, int r0, int step) {
+  for (int r = r0; r < Z; r += step)
+    for (int c0 = 0; c0 < mcol; c0 += 8) {
+      S t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = post[min(c0 + u, mcol - 1) * Z + r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < mcol) g[base + (c0 + u) * gs + r] = t[u];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// edge tables
+// ---------------------------------------------------------------------------
+
+// Each block first copies the code's edge words and layer weights into the front of
+// its LDS ([edges | layer weights]); the row loops then read them with broadcast LDS
+// loads instead of global loads on their critical path.
+__device__ __forceinline__ void tables_to_lds(uint32_t* __restrict__ tab, const uint32_t* __restrict__ edges, int ne,
+                                              const int32_t* __restrict__ lw, int mrow) {
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) tab[i] = edges[i];
+  for (int i = threadIdx.x; i < mrow; i += blockDim.x) tab[ne + i] = (uint32_t)lw[i];
+}
+
+// LDS bytes of that copy for one code, 16-byte aligned
+inline int table_bytes(int nedge, int mrow) { return ((nedge + mrow) * 4 + 15) / 16 * 16; }
+
+// variable index of edge word `ed` for row r: (col*z) + (shift + r) mod z
+__device__ __forceinline__ int edge_var(uint32_t ed, int r, int Z) {
+  int v = (int)(ed & 0xffffu) + r;
+  v = v >= Z ? v - Z : v;
+  return v + (int)(ed >> 16);
+}
+
+// ---------------------------------------------------------------------------
 // one check row, integer elements (min-sum)
 // ---------------------------------------------------------------------------
 
@@ -87,10 +146,7 @@ __device__ __forceinline__ void row_int(typename P::S* __restrict__ post, typena
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
-      const uint32_t ed = edges[e0 + j];
-      int v = (int)(ed & 0xffffu) + r;
-      v = v >= Z ? v - Z : v;
-      v += (int)(ed >> 16);
+      const int v = edge_var(edges[e0 + j], r, Z);
       vi[j] = v;
       const C qq = P::sat((C)post[v] - (C)c2v[(e0 + j) * Z + r]);
       q[j] = qq;
@@ -119,6 +175,17 @@ __device__ __forceinline__ void row_int(typename P::S* __restrict__ post, typena
   }
 }
 
+// parity of row r of a layer (sign bits of its variables)
+template <class P, int MAXW>
+__device__ __forceinline__ bool row_parity(const typename P::S* __restrict__ post, const uint32_t* __restrict__ edges,
+                                           int e0, int w, int r, int Z) {
+  bool par = false;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j)
+    if (j < w) par ^= is_neg<P>(post[edge_var(edges[e0 + j], r, Z)]);
+  return par;
+}
+
 // ---------------------------------------------------------------------------
 // one check row, float elements (min-sum or sum-product)
 // ---------------------------------------------------------------------------
@@ -134,10 +201,7 @@ __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, type
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
-      const uint32_t ed = edges[e0 + j];
-      int v = (int)(ed & 0xffffu) + r;
-      v = v >= Z ? v - Z : v;
-      v += (int)(ed >> 16);
+      const int v = edge_var(edges[e0 + j], r, Z);
       vi[j] = v;
       const F qq = post[v] - c2v[(e0 + j) * Z + r];
       q[j] = qq;
@@ -188,11 +252,14 @@ __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, type
 // the per-wave decode loop: every lane returns its codeword's iteration count
 // ---------------------------------------------------------------------------
 
+// `tab`: the code's [edges | layer weights] copy in LDS
 template <class P, int MAXW>
-__device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S* post, typename P::S* c2v,
-                                                int slot, int lr, bool valid) {
+__device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t* __restrict__ tab,
+                                                typename P::S* post, typename P::S* c2v, int slot, int lr, bool valid) {
   using S = typename P::S;
   const int L = a.lanes_per_cw, Z = a.z;
+  const uint32_t* edges = tab;
+  const int32_t* layer_w = reinterpret_cast<const int32_t*>(tab + a.nedge);
   if (valid)
     for (int i = lr; i < a.e; i += L) c2v[i] = S(0);   // LDPCdecode.cpp:115-119
   wave_sync();
@@ -205,7 +272,7 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
     if (active) {
       int e0 = 0;
       for (int l = 0; l < a.mrow; ++l) {
-        const int w = a.layer_w[l];
+        const int w = layer_w[l];
         S fb = fbeta;
         long long ib = a.ibeta;
         if (a.beta_tab) {   // per-(iteration, layer) offsets
@@ -215,8 +282,8 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
         for (int k = 0; k < a.rows_per_lane; ++k) {
           const int r = lr + k * L;
           if (r < Z) {
-            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, fb, fcap);
-            else row_int<P, MAXW>(post, c2v, a.edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
+            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
+            else row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
           }
         }
         e0 += w;
@@ -224,20 +291,10 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
       }
       e0 = 0;
       for (int l = 0; l < a.mrow; ++l) {                  // LDPCdecode.cpp:172-195
-        const int w = a.layer_w[l];
+        const int w = layer_w[l];
         for (int k = 0; k < a.rows_per_lane; ++k) {
           const int r = lr + k * L;
-          if (r < Z) {
-            bool par = false;
-            for (int j = 0; j < w; ++j) {
-              const uint32_t ed = a.edges[e0 + j];
-              int v = (int)(ed & 0xffffu) + r;
-              v = v >= Z ? v - Z : v;
-              v += (int)(ed >> 16);
-              par ^= is_neg<P>(post[v]);
-            }
-            fail |= par;
-          }
+          if (r < Z) fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
         }
         e0 += w;
       }
@@ -252,14 +309,5 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, typename P::S*
   wave_sync();
   return iter;
 }
-
-// dispatch on the row-weight bucket of a decoder
-#define FECSIM_MAXW_DISPATCH(maxw, KERNEL, P, G, grid, block, lds, st, ...)                 \
-  switch (maxw) {                                                                            \
-  case 8: hipLaunchKernelGGL((KERNEL<P, 8, G>), grid, block, lds, st, __VA_ARGS__); break;   \
-  case 16: hipLaunchKernelGGL((KERNEL<P, 16, G>), grid, block, lds, st, __VA_ARGS__); break; \
-  case 24: hipLaunchKernelGGL((KERNEL<P, 24, G>), grid, block, lds, st, __VA_ARGS__); break; \
-  default: hipLaunchKernelGGL((KERNEL<P, 32, G>), grid, block, lds, st, __VA_ARGS__); break; \
-  }
 
 }  // namespace fecsim

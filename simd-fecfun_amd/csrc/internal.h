@@ -34,8 +34,11 @@ struct DecShape {
   int rows_per_lane = 1, lanes_per_cw = 64, cw_per_wave = 1, waves = 4;
   int maxw = 8;
   int post_stride = 0, c2v_stride = 0;  // elements per slot, padded
-  int wave_bytes = 0;                    // LDS bytes per wave
+  int wave_bytes = 0;                    // LDS bytes per wave (per block when wg)
+  int tab_bytes = 0;                     // LDS bytes of the edge/layer tables in front of the slices
   bool c2v_global = false;               // C2V in a global scratch (codewords too large for LDS)
+  bool wg = false;                       // one codeword per workgroup of wg_threads threads
+  int wg_threads = 0;
 };
 
 // What the decode loop needs (device pointers + parameters)
@@ -43,6 +46,9 @@ struct DecCore {
   const uint32_t* edges;    // per edge: (col*z) << 16 | shift
   const int32_t* layer_w;   // row weight per layer
   int mrow, mcol, z, n, e;  // layers, columns, lifting size, mcol*z, total_hc_ones*z
+  int nedge;                // total_hc_ones
+  int tab_off;              // word offset of this code's [edges | layer_w] in the block's LDS tables
+  int tab_bytes;            // LDS bytes in front of the codeword slices (all codes' tables)
   int lanes_per_cw, cw_per_wave, waves, rows_per_lane;
   int post_stride, c2v_stride, wave_bytes;
   uint32_t max_iter;
@@ -176,6 +182,8 @@ struct MultiArgs {
   uint32_t* item_start;      // [ncode + 1]
   uint32_t* counter;         // next work item
   int wave_bytes, waves;     // LDS per wave (max over codes), waves per block
+  const uint32_t* tabs;      // every code's [edges | layer_w], copied to the front of LDS
+  int tab_words, tab_bytes;
 };
 
 // Layout of the row-pair int8 kernel (pair_i8.hip)
@@ -208,6 +216,8 @@ void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 // blocks of the generic kernel resident on the whole device for decoder d's shape
 int generic_resident_blocks(const DecoderDev& d);
+// raises the kernel's dynamic LDS limit when a workgroup shape needs more than 64 KB
+hipError_t prepare_generic(const DecoderDev& d);
 // device form of a per-(iteration, layer) offset table (long long for integer kinds,
 // the bits of a double for float kinds)
 void offsets_table(const DecoderDev& d, const float* beta, size_t n, std::vector<long long>* out);
@@ -277,6 +287,9 @@ struct fecsim_multi {
   uint32_t grid = 0, max_cw = 0;
   fecsim::DecCore* d_cores = nullptr;
   uint32_t* d_ws = nullptr;               // perm | cnt | cursor | code_start | item_start | counter
+  uint32_t* d_tabs = nullptr;             // all codes' [edges | layer_w]
+  std::vector<int> tab_off;               // word offset of each code's tables
+  int tab_words = 0, tab_bytes = 0;
 };
 
 namespace fecsim {

@@ -58,7 +58,10 @@ __global__ void __launch_bounds__(256) k_multi_decode(const MultiArgs a) {
   using S = typename P::S;
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  char* wbase = smem + (size_t)wave * a.wave_bytes;
+  uint32_t* tabs = reinterpret_cast<uint32_t*>(smem);
+  for (int i = threadIdx.x; i < a.tab_words; i += blockDim.x) tabs[i] = a.tabs[i];
+  __syncthreads();
+  char* wbase = smem + a.tab_bytes + (size_t)wave * a.wave_bytes;
   const S* gin = reinterpret_cast<const S*>(a.in);
   S* gout = reinterpret_cast<S*>(a.out);
   const uint32_t nitems = a.item_start[a.ncode];
@@ -82,10 +85,10 @@ __global__ void __launch_bounds__(256) k_multi_decode(const MultiArgs a) {
     S* c2v = reinterpret_cast<S*>(wbase) + (size_t)CPW * d.post_stride + (size_t)sl * d.c2v_stride;
     const unsigned long long base = valid ? a.off[cw] : 0ull;
     if (valid)
-      for (int i = lr; i < d.n; i += L) post[i] = gin[base + i];
-    const uint32_t iter = decode_loop<P, MAXW>(d, post, c2v, slot, lr, valid);
+      cols_to_lds(post, gin, (long long)base, d.z, d.mcol, d.z, lr, L);
+    const uint32_t iter = decode_loop<P, MAXW>(d, tabs + d.tab_off, post, c2v, slot, lr, valid);
     if (valid) {
-      for (int i = lr; i < d.n; i += L) gout[base + i] = post[i];
+      cols_from_lds(gout, post, (long long)base, d.z, d.mcol, d.z, lr, L);
       if (lr == 0 && a.iters) a.iters[cw] = iter;
     }
     if (a.tot_iter) {
@@ -150,7 +153,7 @@ hipError_t launch_multi(int elem, int mode, int maxw, const MultiArgs& a, uint32
   hipLaunchKernelGGL(k_multi_hist, dim3(g1), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_multi_scan, dim3(1), dim3(64), 0, st, a);
   hipLaunchKernelGGL(k_multi_scatter, dim3(g1), dim3(256), 0, st, a);
-  const size_t lds = (size_t)a.waves * a.wave_bytes;
+  const size_t lds = (size_t)a.tab_bytes + (size_t)a.waves * a.wave_bytes;
   const dim3 g(grid), b(64 * a.waves);
   using F1 = FloatPolicy<float, true>;
   using F0 = FloatPolicy<float, false>;

@@ -17,6 +17,7 @@ namespace {
 void free_multi(fecsim_multi* m) {
   (void)hipFree(m->d_cores);
   (void)hipFree(m->d_ws);
+  (void)hipFree(m->d_tabs);
 }
 
 // recompute the per-code cores, the LDS slice and the persistent grid for m->shapes
@@ -28,14 +29,17 @@ int update(fecsim_multi* m) {
     DecoderDev tmp = m->decs[c]->d;
     tmp.shape = m->shapes[c];
     fill_core(tmp, &cores[c]);
+    cores[c].tab_off = m->tab_off[c];
+    cores[c].tab_bytes = m->tab_bytes;
     wb = std::max(wb, m->shapes[c].wave_bytes);
   }
   m->wave_bytes = wb;
-  m->waves = std::max(1, std::min(m->waves_pref, 65536 / wb));
+  m->waves = std::max(1, std::min(m->waves_pref, std::max(1, 65536 - m->tab_bytes) / wb));
   for (int c = 0; c < n; ++c) cores[c].waves = m->waves;
   const hipError_t e = hipMemcpy(m->d_cores, cores.data(), sizeof(DecCore) * n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "multi-code tables to device");
-  const int blocks = multi_resident_blocks(m->elem, m->mode, m->maxw, m->waves, (size_t)m->waves * wb);
+  const int blocks =
+      multi_resident_blocks(m->elem, m->mode, m->maxw, m->waves, (size_t)m->tab_bytes + (size_t)m->waves * wb);
   if (blocks < 1) return fail(FECSIM_E_HIP, "occupancy query failed");
   m->grid = (uint32_t)blocks;
   return FECSIM_OK;
@@ -62,6 +66,9 @@ MultiArgs args(const fecsim_multi* m, uint32_t num_cw, const int32_t* code, cons
   a.counter = a.item_start + n + 1;
   a.wave_bytes = m->wave_bytes;
   a.waves = m->waves;
+  a.tabs = m->d_tabs;
+  a.tab_words = m->tab_words;
+  a.tab_bytes = m->tab_bytes;
   return a;
 }
 
@@ -138,9 +145,27 @@ int fecsim_multi_create(fecsim_decoder* const* decs, int ncode, uint32_t max_cw,
     m->shapes.push_back(s);
     m->maxw = std::max(m->maxw, s.maxw);
   }
+  // every code's [edge words | layer weights], copied to the front of each block's LDS
+  std::vector<uint32_t> tabs;
+  for (fecsim_decoder* d : m->decs) {
+    const HdecHost& h = d->d.h;
+    m->tab_off.push_back((int)tabs.size());
+    for (int e = 0; e < h.total_hc_ones; ++e)
+      tabs.push_back(((uint32_t)(h.row_active_cols[e] * h.z) << 16) | (uint32_t)h.row_offset[e]);
+    for (int l = 0; l < h.mrow; ++l) tabs.push_back((uint32_t)h.row_weights[l]);
+  }
+  m->tab_words = (int)tabs.size();
+  m->tab_bytes = (m->tab_words * 4 + 15) / 16 * 16;
+  if (m->tab_bytes > 32768) {
+    delete m;
+    return fail(FECSIM_E_UNSUPPORTED, "the codes' edge tables exceed 32 KB of LDS");
+  }
   hipError_t e = hipMalloc((void**)&m->d_cores, sizeof(DecCore) * ncode);
   if (e == hipSuccess) e = hipMalloc((void**)&m->d_ws, sizeof(uint32_t) * ((size_t)max_cw + 4 * (size_t)ncode + 3));
   if (e == hipSuccess) e = hipMemset(m->d_ws, 0, sizeof(uint32_t) * ((size_t)max_cw + 4 * (size_t)ncode + 3));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_tabs, sizeof(uint32_t) * tabs.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(m->d_tabs, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_multi(m);
     delete m;
