@@ -115,6 +115,27 @@ def probe(args):
         print(json.dumps({"probe": "traffic", "ncw": ncw, "copy_bytes": x.numel(),
                           "mean_iterations": float(it.float().mean())}))
         return
+    if args.probe == "host":
+        # fecsim_decode_set_host: LLRs in pageable host memory, decoded in place through
+        # the GPU (copy in, decode, copy out), as a caller holding host buffers sees it
+        bits = henc.new_bits(ncw)
+        henc.gen_codewords(bits, seed=SEED)
+        x = torch.empty((ncw, cs.n), dtype=torch.int8, device="cuda")
+        henc.gen_llr(bits, x, _L.I8, -0.5, seed=SEED)
+        host = x.cpu().numpy()
+        work = host.copy()
+        dec.decode_set_host(work)
+        times = []
+        for _ in range(max(1, args.steps)):
+            work[...] = host
+            t0 = time.perf_counter()
+            dec.decode_set_host(work)
+            times.append(time.perf_counter() - t0)
+        best = min(times)
+        print(json.dumps({"probe": "host", "ncw": ncw, "bytes_each_way": int(host.nbytes),
+                          "seconds_best": best, "cw_per_s": ncw / best,
+                          "info_gbit_s": ncw * K_INFO / best / 1e9}))
+        return
     sim = Sim(henc, dec, ncw, list(SNRS), seed=SEED, use_graph=False)
     sim.run(1)
     torch.cuda.synchronize()
@@ -137,10 +158,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel", choices=["auto", "generic", "pair"], default="auto",
                     help="decoder kernel (results are identical)")
-    ap.add_argument("--probe", choices=["traffic", "sim"], default=None,
+    ap.add_argument("--probe", choices=["traffic", "sim", "host"], default=None,
                     help="short fixed workloads for profiler runs instead of the benchmark: 'traffic' = "
                          "one decode of --ncw codewords at -0.5 dB twice, then a plain copy of the LLR "
-                         "tensor twice (counter calibration); 'sim' = --steps simulation loops")
+                         "tensor twice (counter calibration); 'sim' = --steps simulation loops; 'host' = "
+                         "decode of --ncw codewords held in host memory (PCIe copies included), best of --steps")
     ap.add_argument("--config", type=int, choices=[1, 2, 3, 4, 5], default=1,
                     help="1: 11ad 1/2 int8 decode (the headline metric); 2: 16e 1/2 float sum-product Z sweep; "
                          "3: mixed-code queue; 4: trainable decoder step; 5: 5G NR BG1 Z=384 int8")
