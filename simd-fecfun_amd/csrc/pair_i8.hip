@@ -10,12 +10,14 @@
 //   each variable is stored twice (low half of pair t, high half of pair t - Z/2), so
 //   the lane reads its two rows' variables with one ds_read_b32 and no rotation fix-up,
 //   and writes the pair and its half-swapped twin.
-// * Pair t of column c sits at dword (c*CS + (t mod Z/2) + 32*(t >= Z/2)): a lane group
-//   reads Z/2 consecutive pairs (mod Z), whose t mod Z/2 are distinct, so they hit
-//   distinct banks; slot bases are offset by Z/2 banks so the three slots of a wave
-//   never collide either.
-// * C2V messages (packed pairs) and the per-lane LDS addresses of every edge stay in
-//   VGPRs for the whole decode.
+// * A slot is mcol columns of 64 dwords (slot bases are multiples of 256 bytes); pair t
+//   of column c sits at dword c*64 + ((t mod Z/2) + rot) mod 32 + 32*(t >= Z/2), with
+//   rot = slot * Z/2 (mod 32).  A slot's lanes read Z/2 pairs whose t mod Z/2 are
+//   distinct, so they hit distinct banks, and the rotation keeps the slots of a wave
+//   apart.  The twin of pair t (pair t + Z/2 mod Z) is at the same position with the
+//   other value of the 32-dword bit, i.e. at byte address (address of t) xor 128.
+// * C2V messages (packed pairs) and the per-lane LDS byte address of every edge's pair
+//   stay in VGPRs for the whole decode.
 // Saturating int8 semantics: Q = sat8(L - R) via v_pk_sub + clamp, |Q| as
 // max(Q, -Q) in int16 (so abs(-128) = 128 and |Q| - beta never wraps for 1 <= beta),
 // R = (S xor sign Q) ? -M : M, L = sat8(Q + R).
@@ -32,17 +34,22 @@ __device__ __forceinline__ short2v as_s(uint32_t v) { return __builtin_bit_cast(
 __device__ __forceinline__ ushort2v as_us(uint32_t v) { return __builtin_bit_cast(ushort2v, v); }
 __device__ __forceinline__ uint32_t as_u(ushort2v v) { return __builtin_bit_cast(uint32_t, v); }
 
+// LDS word at an absolute LDS byte address held in a VGPR (no base added per access)
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+__device__ __forceinline__ LdsU32* lds32(uint32_t byte_addr) { return (LdsU32*)(size_t)byte_addr; }
+
 __device__ __forceinline__ short2v smax(short2v a, short2v b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ short2v smin(short2v a, short2v b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ ushort2v umin(ushort2v a, ushort2v b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ ushort2v umax(ushort2v a, ushort2v b) { return __builtin_elementwise_max(a, b); }
 
 struct PairArgs {
-  const uint32_t* adrtab;  // [half][ne]: per lane and edge, LDS byte offsets (in the slot) of the
-                           // pair read and of its twin, packed low | high << 16
+  const uint32_t* adrtab;  // [cw_per_wave][half][ne]: per slot, lane and edge, the LDS byte
+                           // offset (in the slot) of the pair the lane reads
   int z, half, mcol;       // Z, Z/2
   int cw_per_wave, waves;
-  int col_dwords;          // dwords per column in LDS
+  int col_dwords;          // dwords per column in LDS (64; passed at run time so the refill and
+                           // write-out loops are not unrolled into extra live registers)
   int slot_dwords;         // dwords per slot (stride)
   uint32_t num_cw, max_iter;
   long long chunk;         // codewords per wave (persistent grid)
@@ -65,16 +72,15 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
   const int slot = real ? lane / L : 0;
   const int lr = real ? lane - slot * L : 0;
   uint32_t* sbase = smem + (size_t)(wave * CPW + slot) * a.slot_dwords;
+  const int pos = (lr + slot * L) & 31;      // this lane's pair position in a column
 
-  // per-edge LDS byte addresses of the pair this lane reads (t = (s + lr) mod Z) and of
-  // its twin ((t + Z/2) mod Z), both absolute in the block's LDS (< 64 KiB)
-  const uint32_t sb_off = (uint32_t)((wave * CPW + slot) * a.slot_dwords * 4);
-  const uint32_t sb2 = sb_off | (sb_off << 16);
+  // per-edge LDS byte address of the pair this lane reads (t = (s + lr) mod Z), absolute
+  // in the block's LDS; its twin is at address ^ 128
+  const uint32_t sb_off = (uint32_t)(size_t)(LdsU32*)sbase;   // multiple of 256
   uint32_t adr[NE];
-  const uint32_t* at = a.adrtab + (size_t)lr * NE;   // [lane][edge]: immediate offsets
+  const uint32_t* at = a.adrtab + ((size_t)slot * L + lr) * NE;
 #pragma unroll
-  for (int e = 0; e < NE; ++e) adr[e] = at[e] + sb2;
-  char* sb = reinterpret_cast<char*>(smem);
+  for (int e = 0; e < NE; ++e) adr[e] = at[e] + sb_off;
 
   // persistent work: this wave decodes codewords [w0, w1); each slot takes the next one
   // as soon as its current codeword stops
@@ -112,8 +118,8 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           for (int c = 0; c < a.mcol; ++c) {
             const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
             const uint32_t p = (uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16);
-            sbase[c * a.col_dwords + lr] = p;
-            sbase[c * a.col_dwords + lr + 32] = (p >> 16) | (p << 16);
+            sbase[c * a.col_dwords + pos] = p;
+            sbase[c * a.col_dwords + pos + 32] = (p >> 16) | (p << 16);
           }
         }
       }
@@ -132,11 +138,13 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
     for (int l = 0; l < SH::mrow; ++l) {
       constexpr int MAXW = SH::maxw;
       const int w = SH::w[l];
-      // A'_j = max(|Q_j|, beta) = A_j + beta; min1/min2 are tracked on A' and beta is
-      // subtracted once per row.  The exclusive minimum is
-      //   M_j = min(cap, j is the argmin ? min2 : min1) = max(mv1, min(mv2, min1 + min2 - A_j))
-      // with mv = min(cap, min - beta): every A_j other than the argmin is >= min2, so
-      // min1 + min2 - A_j <= min1 there, and it equals min2 at the argmin (ties included).
+      // Min1/min2 are tracked on |Q_j| (in int16, |-128| = 128) and then raised to beta:
+      // m' = max(m, beta) is the minimum of A'_j = max(|Q_j|, beta) = A_j + beta.  With
+      // mv = min(cap, m' - beta) the exclusive minimum is
+      //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, min(mv2, m1' + m2' - beta - |Q_j|)):
+      // at the argmin |Q_j| = m1 <= m1', so the inner term is >= m2' - beta >= mv2; at any
+      // other edge |Q_j| >= m2, so the term is <= m1' - beta when m2 >= beta, and when
+      // m2 < beta then mv1 = mv2 = 0 and every M_j is 0.  Ties included.
       short2v q[MAXW];
       short2v A[MAXW];
       uint32_t S = 0;
@@ -144,20 +152,24 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
-          const uint32_t x = *reinterpret_cast<const uint32_t*>(sb + (adr[e0 + j] & 0xffffu));
+          const uint32_t x = *lds32(adr[e0 + j]);
           short2v qq = as_s(x) - as_s(c2v[e0 + j]);
           qq = smin(smax(qq, lo8), hi8);
           q[j] = qq;
-          const short2v av = smax(smax(qq, beta2), -qq);
+          const short2v av = smax(qq, -qq);
           A[j] = av;
           S ^= as_u(qq);
           m2 = smin(m2, smax(m1, av));
           m1 = smin(m1, av);
         }
       }
+      m1 = smax(m1, beta2);
+      m2 = smax(m2, beta2);
       const short2v mv1 = smin(cap2, m1 - beta2);
       const short2v mv2 = smin(cap2, m2 - beta2);
-      const short2v sumb = m1 + m2 - beta2;
+      uint32_t sumb_u = as_u(m1 + m2 - beta2);
+      asm volatile("" : "+v"(sumb_u));   // one subtraction per edge, not re-associated into two
+      const short2v sumb = as_s(sumb_u);
       // all of the layer's results first, then the stores: the per-edge chains are
       // independent, so the scheduler can interleave them (no back-to-back dependent
       // packed ops waiting on each other)
@@ -177,8 +189,9 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
-          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] & 0xffffu)) = gu[j];
-          *reinterpret_cast<uint32_t*>(sb + (adr[e0 + j] >> 16)) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);
+          const uint32_t ad = adr[e0 + j];
+          *lds32(ad) = gu[j];
+          *lds32(ad ^ 128u) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);   // twin: halves swapped
         }
       }
       e0 += w;
@@ -196,7 +209,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
       uint32_t par = 0;
 #pragma unroll
       for (int j = 0; j < SH::maxw; ++j)
-        if (j < w) par ^= *reinterpret_cast<const uint32_t*>(sb + (adr[e0 + j] & 0xffffu));
+        if (j < w) par ^= *lds32(adr[e0 + j]);
       fail |= (par & 0x80008000u) != 0u;
       e0 += w;
       __builtin_amdgcn_sched_barrier(0);
@@ -208,7 +221,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
     if (finish && real) {
       int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
       for (int c = 0; c < a.mcol; ++c) {
-        const uint32_t p = sbase[c * a.col_dwords + lr];
+        const uint32_t p = sbase[c * a.col_dwords + pos];
         g[c * Z + lr] = (int8_t)(p & 0xff);
         g[c * Z + lr + L] = (int8_t)((p >> 16) & 0xff);
       }
@@ -244,8 +257,8 @@ int pair_i8_shape(const DecoderDev& d) {
   const HdecHost& h = d.h;
   if (d.elem != FECSIM_I8 || d.mode != FECSIM_MIN) return -1;
   if (h.z % 2 != 0 || h.z / 2 > 32 || h.z / 2 < 2) return -1;
-  // one wave's slots must fit the 64 KiB block LDS
-  if ((64 / (h.z / 2)) * (h.mcol * (32 + h.z / 2) + 32) * 4 > 64 * 1024) return -1;
+  // one wave's slots (mcol columns of 64 dwords each) must fit the 64 KiB block LDS
+  if ((64 / (h.z / 2)) * h.mcol * 64 * 4 > 64 * 1024) return -1;
   const float beta = d.params.beta_offset, mrm = d.params.max_row_metric;
   const int b8 = (int)(int8_t)(int)beta;
   if (mrm < 1e-6f) return -1;
@@ -263,6 +276,11 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
   // working through a contiguous chunk of codewords with per-slot refill
   static int resident = -1;
   if (resident < 0) {
+    // the twin addressing (byte address xor 128) needs the slots at multiples of 256
+    // bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH>) != hipSuccess || fa.sharedSizeBytes != 0)
+      return hipErrorInvalidDeviceFunction;
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -284,25 +302,21 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
 void pair_layout(const HdecHost& h, PairLayout* pl) {
   pl->half = h.z / 2;
   pl->cw_per_wave = 64 / pl->half;
-  pl->col_dwords = 32 + pl->half;
-  // slot stride: >= mcol*col_dwords, and = half (mod 32) so slot bases land Z/2 banks apart
-  int sd = h.mcol * pl->col_dwords;
-  while (sd % 32 != pl->half % 32) ++sd;
-  pl->slot_dwords = sd;
-  const int wave_bytes = pl->cw_per_wave * sd * 4;
+  pl->col_dwords = 64;
+  pl->slot_dwords = h.mcol * 64;   // slot bases stay multiples of 256 bytes
+  const int wave_bytes = pl->cw_per_wave * pl->slot_dwords * 4;
   pl->waves = (64 * 1024) / wave_bytes;
   if (pl->waves > 4) pl->waves = 4;
-  const int L = pl->half, Z = h.z;
-  pl->adrtab.assign((size_t)h.total_hc_ones * L, 0u);
-  for (int e = 0; e < h.total_hc_ones; ++e) {
-    const int c = h.row_active_cols[e], s = h.row_offset[e];
-    for (int lr = 0; lr < L; ++lr) {
-      const int t = (s + lr) % Z, tw = (t + L) % Z;
-      const int d0 = c * pl->col_dwords + (t >= L ? t - L + 32 : t);
-      const int d1 = c * pl->col_dwords + (tw >= L ? tw - L + 32 : tw);
-      pl->adrtab[(size_t)lr * h.total_hc_ones + e] = (uint32_t)(d0 * 4) | ((uint32_t)(d1 * 4) << 16);
-    }
-  }
+  const int L = pl->half, Z = h.z, NE = h.total_hc_ones;
+  pl->adrtab.assign((size_t)pl->cw_per_wave * L * NE, 0u);
+  for (int slot = 0; slot < pl->cw_per_wave; ++slot)
+    for (int lr = 0; lr < L; ++lr)
+      for (int e = 0; e < NE; ++e) {
+        const int c = h.row_active_cols[e], s = h.row_offset[e];
+        const int t = (s + lr) % Z;
+        const int d = c * 64 + (((t % L) + slot * L) & 31) + (t >= L ? 32 : 0);
+        pl->adrtab[((size_t)slot * L + lr) * NE + e] = (uint32_t)(d * 4);
+      }
 }
 
 hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t st) {
