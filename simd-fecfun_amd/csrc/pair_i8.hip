@@ -18,9 +18,11 @@
 //   other value of the 32-dword bit, i.e. at byte address (address of t) xor 128.
 // * C2V messages (packed pairs) and the per-lane LDS byte address of every edge's pair
 //   stay in VGPRs for the whole decode.
-// Saturating int8 semantics: Q = sat8(L - R) via v_pk_sub + clamp, |Q| as
-// max(Q, -Q) in int16 (so abs(-128) = 128 and |Q| - beta never wraps for 1 <= beta),
-// R = (S xor sign Q) ? -M : M, L = sat8(Q + R).
+// Values are kept scaled by 256 (an int8 value v is the int16 v << 8): the int16
+// saturating v_pk_add/sub (clamp bit) then saturates exactly at -128 << 8, and one
+// v_pk_min with 127 << 8 finishes the int8 saturation, so Q = sat8(L - R) and
+// L = sat8(Q + R) take two ops each.  |Q| is max(Q, -Q) read as unsigned 16-bit, so
+// |-128 << 8| = 0x8000 = 128 << 8 exactly; R = (S xor sign Q) ? -M : M.
 #include "decode_core.h"
 #include "pair_shapes.h"
 
@@ -37,6 +39,8 @@ __device__ __forceinline__ uint32_t as_u(ushort2v v) { return __builtin_bit_cast
 // LDS word at an absolute LDS byte address held in a VGPR (no base added per access)
 typedef __attribute__((address_space(3))) uint32_t LdsU32;
 __device__ __forceinline__ LdsU32* lds32(uint32_t byte_addr) { return (LdsU32*)(size_t)byte_addr; }
+typedef __attribute__((address_space(3))) uint16_t LdsU16;
+__device__ __forceinline__ LdsU16* lds16(uint32_t byte_addr) { return (LdsU16*)(size_t)byte_addr; }
 
 __device__ __forceinline__ short2v smax(short2v a, short2v b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ short2v smin(short2v a, short2v b) { return __builtin_elementwise_min(a, b); }
@@ -95,9 +99,9 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
   uint32_t c2v[NE];
 #pragma unroll
   for (int e = 0; e < NE; ++e) c2v[e] = 0u;
-  const short2v lo8 = short2v{-128, -128}, hi8 = short2v{127, 127};
-  const short2v beta2 = short2v{(short)a.beta, (short)a.beta};
-  const short2v cap2 = short2v{(short)a.cap, (short)a.cap};
+  const short2v hi8 = short2v{127 << 8, 127 << 8};                       // scaled int8 maximum
+  const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
+  const short2v cap2 = short2v{(short)(a.cap << 8), (short)(a.cap << 8)};
   const unsigned long long gmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
   unsigned long long my_iters = 0;   // summed by lane 0 of each slot
 
@@ -117,7 +121,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
           for (int c = 0; c < a.mcol; ++c) {
             const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
-            const uint32_t p = (uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16);
+            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
             sbase[c * a.col_dwords + pos] = p;
             sbase[c * a.col_dwords + pos + 32] = (p >> 16) | (p << 16);
           }
@@ -138,35 +142,35 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
     for (int l = 0; l < SH::mrow; ++l) {
       constexpr int MAXW = SH::maxw;
       const int w = SH::w[l];
-      // Min1/min2 are tracked on |Q_j| (in int16, |-128| = 128) and then raised to beta:
+      // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128) and then raised to beta:
       // m' = max(m, beta) is the minimum of A'_j = max(|Q_j|, beta) = A_j + beta.  With
       // mv = min(cap, m' - beta) the exclusive minimum is
       //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, min(mv2, m1' + m2' - beta - |Q_j|)):
       // at the argmin |Q_j| = m1 <= m1', so the inner term is >= m2' - beta >= mv2; at any
       // other edge |Q_j| >= m2, so the term is <= m1' - beta when m2 >= beta, and when
-      // m2 < beta then mv1 = mv2 = 0 and every M_j is 0.  Ties included.
+      // m2 < beta then mv1 = mv2 = 0 and every M_j is 0.  Ties included.  The inner term
+      // lies in [beta - 128, 128 - beta] (scaled), so it is exact as a wrapped int16.
       short2v q[MAXW];
-      short2v A[MAXW];
+      ushort2v A[MAXW];
       uint32_t S = 0;
-      short2v m1 = short2v{255, 255}, m2 = short2v{255, 255};
+      ushort2v m1 = ushort2v{0xffff, 0xffff}, m2 = ushort2v{0xffff, 0xffff};
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
           const uint32_t x = *lds32(adr[e0 + j]);
-          short2v qq = as_s(x) - as_s(c2v[e0 + j]);
-          qq = smin(smax(qq, lo8), hi8);
+          const short2v qq = smin(__builtin_elementwise_sub_sat(as_s(x), as_s(c2v[e0 + j])), hi8);
           q[j] = qq;
-          const short2v av = smax(qq, -qq);
+          const ushort2v av = as_us(as_u(smax(qq, -qq)));
           A[j] = av;
           S ^= as_u(qq);
-          m2 = smin(m2, smax(m1, av));
-          m1 = smin(m1, av);
+          m2 = umin(m2, umax(m1, av));
+          m1 = umin(m1, av);
         }
       }
-      m1 = smax(m1, beta2);
-      m2 = smax(m2, beta2);
-      const short2v mv1 = smin(cap2, m1 - beta2);
-      const short2v mv2 = smin(cap2, m2 - beta2);
+      m1 = umax(m1, beta2);
+      m2 = umax(m2, beta2);
+      const short2v mv1 = smin(cap2, as_s(as_u(m1 - beta2)));
+      const short2v mv2 = smin(cap2, as_s(as_u(m2 - beta2)));
       uint32_t sumb_u = as_u(m1 + m2 - beta2);
       asm volatile("" : "+v"(sumb_u));   // one subtraction per edge, not re-associated into two
       const short2v sumb = as_s(sumb_u);
@@ -177,11 +181,10 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
-          const short2v m = smax(mv1, smin(mv2, sumb - A[j]));   // j == argmin ? min2 : min1, capped
+          const short2v m = smax(mv1, smin(mv2, sumb - as_s(as_u(A[j]))));   // argmin ? min2 : min1, capped
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
-          short2v g = q[j] + as_s(rn);
-          g = smin(smax(g, lo8), hi8);
+          const short2v g = smin(__builtin_elementwise_add_sat(q[j], as_s(rn)), hi8);
           c2v[e0 + j] = rn;
           gu[j] = as_u(g);
         }
@@ -191,7 +194,10 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
         if (j < w) {
           const uint32_t ad = adr[e0 + j];
           *lds32(ad) = gu[j];
-          *lds32(ad ^ 128u) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);   // twin: halves swapped
+          // the twin holds the halves swapped: two 16-bit stores (kept apart by the
+          // nontemporal hint, which LDS ignores) instead of a rotate and one store
+          __builtin_nontemporal_store((uint16_t)gu[j], lds16((ad ^ 128u) + 2));
+          __builtin_nontemporal_store((uint16_t)(gu[j] >> 16), lds16(ad ^ 128u));
         }
       }
       e0 += w;
@@ -222,8 +228,8 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
       int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
       for (int c = 0; c < a.mcol; ++c) {
         const uint32_t p = sbase[c * a.col_dwords + pos];
-        g[c * Z + lr] = (int8_t)(p & 0xff);
-        g[c * Z + lr + L] = (int8_t)((p >> 16) & 0xff);
+        g[c * Z + lr] = (int8_t)(p >> 8);
+        g[c * Z + lr + L] = (int8_t)(p >> 24);
       }
       if (a.c2v) {
         // opaque row offset: keeps the NE store offsets from being hoisted out of the
@@ -233,8 +239,8 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
         int8_t* gc = a.c2v + cw * (long long)(NE * Z) + lr;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-          gc[e * zo] = (int8_t)(c2v[e] & 0xff);
-          gc[e * zo + L] = (int8_t)((c2v[e] >> 16) & 0xff);
+          gc[e * zo] = (int8_t)(c2v[e] >> 8);
+          gc[e * zo + L] = (int8_t)(c2v[e] >> 24);
         }
       }
       if (lr == 0) {
