@@ -188,7 +188,7 @@ struct MultiArgs {
 
 // Layout of the row-pair int8 kernel (pair_i8.hip)
 struct PairLayout {
-  int half = 0, cw_per_wave = 0, col_dwords = 0, slot_dwords = 0, waves = 0;
+  int half = 0, cw_per_wave = 0, col_dwords = 0, wave_dwords = 0, waves = 0;
   std::vector<uint32_t> adrtab;   // [half][total_hc_ones]
 };
 

@@ -10,14 +10,13 @@
 //   each variable is stored twice (low half of pair t, high half of pair t - Z/2), so
 //   the lane reads its two rows' variables with one ds_read_b32 and no rotation fix-up,
 //   and writes the pair and its half-swapped twin.
-// * A slot is mcol columns of 64 dwords (slot bases are multiples of 256 bytes); pair t
-//   of column c sits at dword c*64 + ((t mod Z/2) + rot) mod 32 + 32*(t >= Z/2), with
-//   rot = slot * Z/2 (mod 32).  A slot's lanes read Z/2 pairs whose t mod Z/2 are
-//   distinct, so they hit distinct banks, and the rotation keeps the slots of a wave
-//   apart.  The twin of pair t (pair t + Z/2 mod Z) is at the same position with the
-//   other value of the 32-dword bit, i.e. at byte address (address of t) xor 128.
-// * C2V messages (packed pairs) and the per-lane LDS byte address of every edge's pair
-//   stay in VGPRs for the whole decode.
+// * The slots of a wave share its LDS columns: column c of a wave is two rows of 64
+//   dwords (wave bases are multiples of 512 bytes); pair t of slot s sits in row
+//   (t >= Z/2) at dword (t mod Z/2) + s*Z/2, so a column holds all slots' 2*Z/2 pairs
+//   and the twin of pair t (pair t + Z/2 mod Z) is at byte address (address of t) xor
+//   256.  A slot's lanes read Z/2 pairs with distinct t mod Z/2, hence distinct banks.
+// * The per-lane LDS byte address of every edge's pair stays in a VGPR for the whole
+//   decode; the C2V messages too, as int8 bytes, two edges per VGPR.
 // Values are kept scaled by 256 (an int8 value v is the int16 v << 8): the int16
 // saturating v_pk_add/sub (clamp bit) then saturates exactly at -128 << 8, and one
 // v_pk_min with 127 << 8 finishes the int8 saturation, so Q = sat8(L - R) and
@@ -42,6 +41,17 @@ __device__ __forceinline__ LdsU32* lds32(uint32_t byte_addr) { return (LdsU32*)(
 typedef __attribute__((address_space(3))) uint16_t LdsU16;
 __device__ __forceinline__ LdsU16* lds16(uint32_t byte_addr) { return (LdsU16*)(size_t)byte_addr; }
 
+// C2V message pair of edge e (row lr, row lr + Z/2) as scaled int16 halves, from the
+// packed bytes; and the store back.  e is a compile-time index in the unrolled loops.
+template <int N>
+__device__ __forceinline__ uint32_t c2v_get(const uint32_t (&p)[N], int e) {
+  return (e & 1) ? (p[e >> 1] & 0xff00ff00u) : __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu);
+}
+template <int N>
+__device__ __forceinline__ void c2v_put(uint32_t (&p)[N], int e, uint32_t r) {
+  p[e >> 1] = (e & 1) ? __builtin_amdgcn_perm(r, p[e >> 1], 0x07020500u) : __builtin_amdgcn_perm(p[e >> 1], r, 0x07030501u);
+}
+
 __device__ __forceinline__ short2v smax(short2v a, short2v b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ short2v smin(short2v a, short2v b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ ushort2v umin(ushort2v a, ushort2v b) { return __builtin_elementwise_min(a, b); }
@@ -49,12 +59,12 @@ __device__ __forceinline__ ushort2v umax(ushort2v a, ushort2v b) { return __buil
 
 struct PairArgs {
   const uint32_t* adrtab;  // [cw_per_wave][half][ne]: per slot, lane and edge, the LDS byte
-                           // offset (in the slot) of the pair the lane reads
+                           // offset (in the wave's columns) of the pair the lane reads
   int z, half, mcol;       // Z, Z/2
   int cw_per_wave, waves;
-  int col_dwords;          // dwords per column in LDS (64; passed at run time so the refill and
+  int col_dwords;          // dwords per wave column (128; passed at run time so the refill and
                            // write-out loops are not unrolled into extra live registers)
-  int slot_dwords;         // dwords per slot (stride)
+  int wave_dwords;         // dwords per wave (mcol * col_dwords)
   uint32_t num_cw, max_iter;
   long long chunk;         // codewords per wave (persistent grid)
   int max_blocks;          // resident-grid size bound (0: one block per cw_per_wave*waves codewords)
@@ -66,8 +76,16 @@ struct PairArgs {
   unsigned long long* tot_iter;
 };
 
+// waves per SIMD the register allocation aims at: 4 when the edge addresses, the packed
+// C2V bytes and a layer's working set fit 128 VGPRs, else no constraint
 template <class SH>
-__global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
+constexpr int pair_waves_per_eu() {
+  return (SH::ne + 1) / 2 + SH::ne + 3 * SH::maxw + 20 <= 128 ? 4 : 1;
+}
+
+template <class SH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<SH>())))
+k_pair_i8(const PairArgs a) {
   constexpr int NE = SH::ne;
   HIP_DYNAMIC_SHARED(uint32_t, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -75,16 +93,16 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
   const bool real = lane < CPW * L;          // lanes past the last slot mirror lane 0
   const int slot = real ? lane / L : 0;
   const int lr = real ? lane - slot * L : 0;
-  uint32_t* sbase = smem + (size_t)(wave * CPW + slot) * a.slot_dwords;
-  const int pos = (lr + slot * L) & 31;      // this lane's pair position in a column
+  uint32_t* wbase = smem + (size_t)wave * a.wave_dwords;
+  const int pos = lr + slot * L;             // this lane's pair position in a column row
 
   // per-edge LDS byte address of the pair this lane reads (t = (s + lr) mod Z), absolute
-  // in the block's LDS; its twin is at address ^ 128
-  const uint32_t sb_off = (uint32_t)(size_t)(LdsU32*)sbase;   // multiple of 256
+  // in the block's LDS; its twin is at address ^ 256
+  const uint32_t wb_off = (uint32_t)(size_t)(LdsU32*)wbase;   // multiple of 512
   uint32_t adr[NE];
   const uint32_t* at = a.adrtab + ((size_t)slot * L + lr) * NE;
 #pragma unroll
-  for (int e = 0; e < NE; ++e) adr[e] = at[e] + sb_off;
+  for (int e = 0; e < NE; ++e) adr[e] = at[e] + wb_off;
 
   // persistent work: this wave decodes codewords [w0, w1); each slot takes the next one
   // as soon as its current codeword stops
@@ -96,9 +114,10 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
   long long cw = -1;                 // codeword of this lane's slot, -1 when idle
   bool active = false;
   uint32_t iter = 0;
-  uint32_t c2v[NE];
+  // C2V of edges 2k and 2k+1, int8 bytes [2k row lr, 2k+1 row lr, 2k row lr+L, 2k+1 row lr+L]
+  uint32_t c2vp[(NE + 1) / 2];
 #pragma unroll
-  for (int e = 0; e < NE; ++e) c2v[e] = 0u;
+  for (int k = 0; k < (NE + 1) / 2; ++k) c2vp[k] = 0u;
   const short2v hi8 = short2v{127 << 8, 127 << 8};                       // scaled int8 maximum
   const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
   const short2v cap2 = short2v{(short)(a.cap << 8), (short)(a.cap << 8)};
@@ -122,14 +141,14 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           for (int c = 0; c < a.mcol; ++c) {
             const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
             const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
-            sbase[c * a.col_dwords + pos] = p;
-            sbase[c * a.col_dwords + pos + 32] = (p >> 16) | (p << 16);
+            wbase[c * a.col_dwords + pos] = p;
+            wbase[c * a.col_dwords + pos + 64] = (p >> 16) | (p << 16);
           }
         }
       }
       const uint32_t keep = take ? 0u : ~0u;   // fresh codeword: C2V messages start at zero
 #pragma unroll
-      for (int e = 0; e < NE; ++e) c2v[e] &= keep;
+      for (int k = 0; k < (NE + 1) / 2; ++k) c2vp[k] &= keep;
     }
     if (__ballot(active) == 0ull) break;
     wave_sync();
@@ -158,7 +177,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
           const uint32_t x = *lds32(adr[e0 + j]);
-          const short2v qq = smin(__builtin_elementwise_sub_sat(as_s(x), as_s(c2v[e0 + j])), hi8);
+          const short2v qq = smin(__builtin_elementwise_sub_sat(as_s(x), as_s(c2v_get(c2vp, e0 + j))), hi8);
           q[j] = qq;
           const ushort2v av = as_us(as_u(smax(qq, -qq)));
           A[j] = av;
@@ -185,7 +204,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
           const short2v g = smin(__builtin_elementwise_add_sat(q[j], as_s(rn)), hi8);
-          c2v[e0 + j] = rn;
+          c2v_put(c2vp, e0 + j, rn);
           gu[j] = as_u(g);
         }
       }
@@ -196,8 +215,8 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
           *lds32(ad) = gu[j];
           // the twin holds the halves swapped: two 16-bit stores (kept apart by the
           // nontemporal hint, which LDS ignores) instead of a rotate and one store
-          __builtin_nontemporal_store((uint16_t)gu[j], lds16((ad ^ 128u) + 2));
-          __builtin_nontemporal_store((uint16_t)(gu[j] >> 16), lds16(ad ^ 128u));
+          __builtin_nontemporal_store((uint16_t)gu[j], lds16((ad ^ 256u) + 2));
+          __builtin_nontemporal_store((uint16_t)(gu[j] >> 16), lds16(ad ^ 256u));
         }
       }
       e0 += w;
@@ -227,7 +246,7 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
     if (finish && real) {
       int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
       for (int c = 0; c < a.mcol; ++c) {
-        const uint32_t p = sbase[c * a.col_dwords + pos];
+        const uint32_t p = wbase[c * a.col_dwords + pos];
         g[c * Z + lr] = (int8_t)(p >> 8);
         g[c * Z + lr + L] = (int8_t)(p >> 24);
       }
@@ -239,8 +258,9 @@ __global__ void __launch_bounds__(256) k_pair_i8(const PairArgs a) {
         int8_t* gc = a.c2v + cw * (long long)(NE * Z) + lr;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-          gc[e * zo] = (int8_t)(c2v[e] >> 8);
-          gc[e * zo + L] = (int8_t)(c2v[e] >> 24);
+          const uint32_t r = c2v_get(c2vp, e);
+          gc[e * zo] = (int8_t)(r >> 8);
+          gc[e * zo + L] = (int8_t)(r >> 24);
         }
       }
       if (lr == 0) {
@@ -263,8 +283,8 @@ int pair_i8_shape(const DecoderDev& d) {
   const HdecHost& h = d.h;
   if (d.elem != FECSIM_I8 || d.mode != FECSIM_MIN) return -1;
   if (h.z % 2 != 0 || h.z / 2 > 32 || h.z / 2 < 2) return -1;
-  // one wave's slots (mcol columns of 64 dwords each) must fit the 64 KiB block LDS
-  if ((64 / (h.z / 2)) * h.mcol * 64 * 4 > 64 * 1024) return -1;
+  // one wave's columns (two rows of 64 dwords each) must fit the 64 KiB block LDS
+  if (h.mcol * 128 * 4 > 64 * 1024) return -1;
   const float beta = d.params.beta_offset, mrm = d.params.max_row_metric;
   const int b8 = (int)(int8_t)(int)beta;
   if (mrm < 1e-6f) return -1;
@@ -277,13 +297,13 @@ template <class SH>
 static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
   PairArgs a = in;
   if (a.num_cw == 0) return hipSuccess;
-  const size_t lds = (size_t)a.waves * a.cw_per_wave * a.slot_dwords * 4;
+  const size_t lds = (size_t)a.waves * a.wave_dwords * 4;
   // persistent grid: as many blocks as stay resident (x2 to smooth the tail), each wave
   // working through a contiguous chunk of codewords with per-slot refill
   static int resident = -1;
   if (resident < 0) {
-    // the twin addressing (byte address xor 128) needs the slots at multiples of 256
-    // bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
+    // the twin addressing (byte address xor 256) needs the waves' columns at multiples
+    // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH>) != hipSuccess || fa.sharedSizeBytes != 0)
       return hipErrorInvalidDeviceFunction;
@@ -308,9 +328,9 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
 void pair_layout(const HdecHost& h, PairLayout* pl) {
   pl->half = h.z / 2;
   pl->cw_per_wave = 64 / pl->half;
-  pl->col_dwords = 64;
-  pl->slot_dwords = h.mcol * 64;   // slot bases stay multiples of 256 bytes
-  const int wave_bytes = pl->cw_per_wave * pl->slot_dwords * 4;
+  pl->col_dwords = 128;
+  pl->wave_dwords = h.mcol * 128;   // wave bases stay multiples of 512 bytes
+  const int wave_bytes = pl->wave_dwords * 4;
   pl->waves = (64 * 1024) / wave_bytes;
   if (pl->waves > 4) pl->waves = 4;
   const int L = pl->half, Z = h.z, NE = h.total_hc_ones;
@@ -320,7 +340,7 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
       for (int e = 0; e < NE; ++e) {
         const int c = h.row_active_cols[e], s = h.row_offset[e];
         const int t = (s + lr) % Z;
-        const int d = c * 64 + (((t % L) + slot * L) & 31) + (t >= L ? 32 : 0);
+        const int d = c * 128 + (t >= L ? 64 : 0) + (t % L) + slot * L;
         pl->adrtab[((size_t)slot * L + lr) * NE + e] = (uint32_t)(d * 4);
       }
 }
@@ -335,7 +355,7 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   a.mcol = h.mcol;
   a.cw_per_wave = pl.cw_per_wave;
   a.col_dwords = pl.col_dwords;
-  a.slot_dwords = pl.slot_dwords;
+  a.wave_dwords = pl.wave_dwords;
   a.waves = pl.waves;
   a.num_cw = ka.num_cw;
   a.max_iter = d.params.max_iter;
