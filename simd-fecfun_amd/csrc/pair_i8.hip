@@ -197,6 +197,7 @@ k_pair_i8(const PairArgs a) {
       // independent, so the scheduler can interleave them (no back-to-back dependent
       // packed ops waiting on each other)
       uint32_t gu[MAXW];
+      uint32_t pend = 0;   // new C2V of an even edge whose odd partner follows in this layer
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
@@ -204,7 +205,10 @@ k_pair_i8(const PairArgs a) {
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
           const short2v g = smin(__builtin_elementwise_add_sat(q[j], as_s(rn)), hi8);
-          c2v_put(c2vp, e0 + j, rn);
+          const int e = e0 + j;   // compile-time after unrolling
+          if ((e & 1) == 0 && j + 1 < w) pend = rn;
+          else if ((e & 1) == 1 && j > 0) c2vp[e >> 1] = __builtin_amdgcn_perm(rn, pend, 0x07030501u);   // both bytes new
+          else c2v_put(c2vp, e, rn);
           gu[j] = as_u(g);
         }
       }
@@ -213,10 +217,7 @@ k_pair_i8(const PairArgs a) {
         if (j < w) {
           const uint32_t ad = adr[e0 + j];
           *lds32(ad) = gu[j];
-          // the twin holds the halves swapped: two 16-bit stores (kept apart by the
-          // nontemporal hint, which LDS ignores) instead of a rotate and one store
-          __builtin_nontemporal_store((uint16_t)gu[j], lds16((ad ^ 256u) + 2));
-          __builtin_nontemporal_store((uint16_t)(gu[j] >> 16), lds16(ad ^ 256u));
+          *lds32(ad ^ 256u) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);   // twin: halves swapped
         }
       }
       e0 += w;
