@@ -19,11 +19,18 @@ namespace fecsim {
 
 namespace {
 
-__global__ void k_multi_hist(MultiArgs a) {
+// Block-private counts in LDS, then one global atomic per code and block (the queue has
+// few codes, so per-entry global atomics would all contend on a handful of words).
+__global__ void __launch_bounds__(256) k_multi_hist(MultiArgs a) {
+  __shared__ uint32_t cnt[64];
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
+  __syncthreads();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.num_cw; i += gridDim.x * blockDim.x) {
     const int c = a.code[i];
-    if (c >= 0 && c < a.ncode) atomicAdd(&a.cnt[c], 1u);   // unknown code ids are skipped
+    if (c >= 0 && c < a.ncode) atomicAdd(&cnt[c], 1u);   // unknown code ids are skipped
   }
+  __syncthreads();
+  if ((int)threadIdx.x < a.ncode && cnt[threadIdx.x]) atomicAdd(&a.cnt[threadIdx.x], cnt[threadIdx.x]);
 }
 
 __global__ void k_multi_scan(MultiArgs a) {
@@ -44,12 +51,24 @@ __global__ void k_multi_scan(MultiArgs a) {
   *a.counter = 0;
 }
 
-__global__ void k_multi_scatter(MultiArgs a) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.num_cw; i += gridDim.x * blockDim.x) {
-    const int c = a.code[i];
-    if (c < 0 || c >= a.ncode) continue;
-    const uint32_t pos = atomicAdd(&a.cursor[c], 1u);
-    a.perm[a.code_start[c] + pos] = i;
+// Per tile of 256 entries: ranks within the tile from LDS atomics, one global atomic per
+// code reserves the tile's range of that code's segment.  The order inside a segment is
+// arbitrary; every codeword decodes the same wherever it sits.
+__global__ void __launch_bounds__(256) k_multi_scatter(MultiArgs a) {
+  __shared__ uint32_t cnt[64], base[64];
+  for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < a.num_cw; t0 += gridDim.x * blockDim.x) {   // uniform per block
+    if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = t0 + threadIdx.x;
+    const int c = i < a.num_cw ? a.code[i] : -1;
+    const bool ok = c >= 0 && c < a.ncode;
+    const uint32_t rank = ok ? atomicAdd(&cnt[c], 1u) : 0u;
+    __syncthreads();
+    if ((int)threadIdx.x < a.ncode && cnt[threadIdx.x])
+      base[threadIdx.x] = atomicAdd(&a.cursor[threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (ok) a.perm[a.code_start[c] + base[c] + rank] = i;
+    __syncthreads();
   }
 }
 
