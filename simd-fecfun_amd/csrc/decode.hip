@@ -158,7 +158,16 @@ static bool fit_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, i
   while (s->cw_per_wave > 1 && s->cw_per_wave * per_slot > room) --s->cw_per_wave;
   s->wave_bytes = s->cw_per_wave * per_slot;
   if (s->wave_bytes > room) return false;
-  s->waves = std::min(4, room / s->wave_bytes);
+  // waves per block: most waves resident in a CU's 160 KB (at most 32); ties -> larger blocks
+  int best = 0;
+  for (int w = 1; w <= std::min(4, room / s->wave_bytes); ++w) {
+    const int blocks = 160 * 1024 / (s->tab_bytes + w * s->wave_bytes);
+    const int resident = std::min(32, blocks * w);
+    if (resident >= best) {
+      best = resident;
+      s->waves = w;
+    }
+  }
   return true;
 }
 
@@ -171,33 +180,26 @@ static double lanes_per_cu(const DecShape& s) {
   return (double)waves * s.cw_per_wave * s.lanes_per_cw;
 }
 
-int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why) {
-  s->maxw = maxw_bucket(h.max_row_weight);
-  if (!s->maxw) { *why = "row weight above 32"; return FECSIM_E_UNSUPPORTED; }
-  if (h.mcol * h.z > 65535) { *why = "mcol*z above 65535"; return FECSIM_E_UNSUPPORTED; }
-  if (h.z > 1024) { *why = "z above 1024"; return FECSIM_E_UNSUPPORTED; }
-  // rows per lane K: most row work in flight per CU once the LDS limits are applied;
-  // ties -> fewer rows per lane.
-  {
-    double best = -1.0;
-    DecShape pick;
-    for (int K = 1; K <= 16; ++K) {
-      DecShape t;
-      t.maxw = s->maxw;
-      if (!fit_shape(h, elem_bytes, extra_slot_bytes, K, &t)) continue;
-      const double sc = lanes_per_cu(t);
-      if (sc > best * (1.0 + 1e-9)) {
-        best = sc;
-        pick = t;
-      }
-    }
-    if (best > 0) {
-      *s = pick;
-      return FECSIM_OK;
+// Best per-wave shape: rows per lane K giving the most row work in flight per CU once
+// the LDS limits are applied; ties -> fewer rows per lane.  Returns its score (0: none).
+static double best_wave_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, int maxw, DecShape* s) {
+  double best = 0.0;
+  for (int K = 1; K <= 16; ++K) {
+    DecShape t;
+    t.maxw = maxw;
+    if (!fit_shape(h, elem_bytes, extra_slot_bytes, K, &t)) continue;
+    const double sc = lanes_per_cu(t);
+    if (sc > best * (1.0 + 1e-9)) {
+      best = sc;
+      *s = t;
     }
   }
-  // too large for a wave's slice: one codeword per workgroup, C2V in LDS when the
-  // block's LDS holds it beside the posteriors, else in global memory
+  return best;
+}
+
+// One codeword per workgroup: C2V in LDS when the block's LDS holds it beside the
+// posteriors, else in global memory.  Returns the rows in flight per CU (0: does not fit).
+static double wg_shape(const HdecHost& h, int elem_bytes, int maxw, DecShape* s) {
   int lds_max = 65536;
   {
     int dev = 0, v = 0;
@@ -208,7 +210,7 @@ int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecSha
   const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;
   const int pad = 16 / elem_bytes;
   DecShape t;
-  t.maxw = s->maxw;
+  t.maxw = maxw;
   t.post_stride = (N + pad - 1) / pad * pad;
   t.c2v_stride = (E + pad - 1) / pad * pad;
   t.tab_bytes = table_bytes(h.total_hc_ones, h.mrow);
@@ -221,21 +223,42 @@ int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecSha
   const long long posts = (long long)t.post_stride * elem_bytes;
   const long long both = (long long)(t.post_stride + t.c2v_stride) * elem_bytes;
   lds_max -= t.tab_bytes;
-  if (posts > lds_max) {
-    *why = "codeword posteriors do not fit a workgroup's LDS";
-    return FECSIM_E_UNSUPPORTED;
-  }
+  if (posts > lds_max) return 0.0;
   t.c2v_global = both > lds_max;
   t.wave_bytes = (int)(t.c2v_global ? posts : both);
   *s = t;
+  const long long blocks = std::max(1LL, std::min(160LL * 1024 / (t.tab_bytes + t.wave_bytes), 32LL / t.waves));
+  return (double)blocks * std::min(Z, t.wg_threads);
+}
+
+int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why) {
+  s->maxw = maxw_bucket(h.max_row_weight);
+  if (!s->maxw) { *why = "row weight above 32"; return FECSIM_E_UNSUPPORTED; }
+  if (h.mcol * h.z > 65535) { *why = "mcol*z above 65535"; return FECSIM_E_UNSUPPORTED; }
+  if (h.z > 1024) { *why = "z above 1024"; return FECSIM_E_UNSUPPORTED; }
+  DecShape wave, wg;
+  const double sw = best_wave_shape(h, elem_bytes, extra_slot_bytes, s->maxw, &wave);
+  const double sg = wg_shape(h, elem_bytes, s->maxw, &wg);
+  // a workgroup per codeword pays a block barrier per layer: take it when no wave slice
+  // holds the codeword, or when it keeps clearly more rows in flight (codewords so large
+  // that a CU's LDS holds only a few, each on a partly idle wave)
+  if (sw > 0.0 && (sg == 0.0 || wg.c2v_global || sg < 1.25 * sw)) {
+    *s = wave;
+    return FECSIM_OK;
+  }
+  if (sg == 0.0) {
+    *why = "codeword posteriors do not fit a workgroup's LDS";
+    return FECSIM_E_UNSUPPORTED;
+  }
+  *s = wg;
   return FECSIM_OK;
 }
 
 int shape_with_rows(const HdecHost& h, int elem_bytes, int K, DecShape* s) {
-  if (K == 0) {
-    std::string why;
-    const int rc = choose_shape(h, elem_bytes, 0, s, &why);
-    return rc || s->c2v_global || s->wg;
+  if (K == 0) {   // the per-wave choice of choose_shape (workgroup shapes excluded)
+    s->maxw = maxw_bucket(h.max_row_weight);
+    if (!s->maxw || h.mcol * h.z > 65535) return 1;
+    return best_wave_shape(h, elem_bytes, 0, s->maxw, s) > 0.0 ? 0 : 1;
   }
   s->maxw = maxw_bucket(h.max_row_weight);
   const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;

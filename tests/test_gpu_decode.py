@@ -70,7 +70,8 @@ LIVE = [
     ("16e 2/3A", 1, "Vec64c_min", 1.0, 100, 96),
     ("11n 1/2 z81", 1, "Vec64c_min", -0.5, 200, None),
     ("16e 1/2", 1, "Vec16f_prod", -1.0, 60, 48),
-    ("16e 1/2", 1, "Vec16f_prod", -0.5, 60, 96),
+    ("16e 1/2", 1, "Vec16f_prod", -0.5, 60, 96),   # float codewords this large: workgroup per codeword
+    ("16e 1/2", 1, "Vec16f_min", -0.5, 60, 68),
     ("11n 1/2 z81", 1, "Vec16f_min", -1.0, 60, None),
     ("11ad 1/2", 1, "Vec32s_min", -0.5, 100, None),
 ]
