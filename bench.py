@@ -37,7 +37,7 @@ OPS_PER_EDGE_UPDATE = 13                    # algorithmic int ops per edge-updat
 E_11AD = 2184                               # edges per iteration, 11ad 1/2
 # HBM bytes per codeword moved by one k_pair_i8 launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs scaled by a calibration copy (profiles/k_pair_i8_11ad12_1e6cw_hbm_traffic.md)
-PAIR_TRAFFIC_BYTES_PER_CW = 1354.1
+PAIR_TRAFFIC_BYTES_PER_CW = 1358.0
 
 
 def make_inputs(henc, ncw, seed, torch, llr_scale=2.0, max_abs=24.0):
