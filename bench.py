@@ -38,6 +38,8 @@ E_11AD = 2184                               # edges per iteration, 11ad 1/2
 # HBM bytes per codeword moved by one k_pair_i8 launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs scaled by a calibration copy (profiles/k_pair_i8_11ad12_1e6cw_hbm_traffic.md)
 PAIR_TRAFFIC_BYTES_PER_CW = 1358.0
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12     # 256 CUs x 128 B/clk x 2.4 GHz
+LDS_BYTES_PER_EDGE_UPDATE = 8
 
 
 def make_inputs(henc, ncw, seed, torch, llr_scale=2.0, max_abs=24.0):
@@ -236,6 +238,7 @@ def main():
     hbm_gbs = hbm_bytes / (avg_launch_ms * 1e-3) / 1e9
     edge_updates_per_launch = tot_iter / len(SNRS) * E_11AD
     valu_tops = edge_updates_per_launch * OPS_PER_EDGE_UPDATE / (avg_launch_ms * 1e-3) / 1e12
+    lds_tbs = edge_updates_per_launch * LDS_BYTES_PER_EDGE_UPDATE / (avg_launch_ms * 1e-3) / 1e12
     mean_iters = {"%g" % s: float(it.float().mean().item()) for s, it in zip(SNRS, iters)}
 
     # error rates of the decoded batch (data columns vs transmitted bits)
@@ -285,6 +288,11 @@ def main():
         "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                           "frac": valu_tops / VALU_PEAK_TOPS,
                           "ops_per_edge_update": OPS_PER_EDGE_UPDATE, "edge_updates_per_launch": edge_updates_per_launch},
+        # LDS bytes the row-pair kernel moves per edge-update (one row of one edge, one iteration):
+        # its pair read, pair + twin write and syndrome read are 16 bytes per lane for two rows
+        "roofline_lds": {"bound": "lds", "achieved": lds_tbs, "peak": LDS_PEAK_TBS, "unit": "TB/s",
+                         "frac": lds_tbs / LDS_PEAK_TBS, "bytes_per_edge_update": LDS_BYTES_PER_EDGE_UPDATE,
+                         "note": "algorithmic bytes; 46% of LDS-active cycles are bank conflicts (32-bank passes)"},
         "learned_offset_gain_db": None,
     }
     if not args.no_cpu:
