@@ -91,6 +91,28 @@ def cpu_baseline(cs, llr_samples, max_seconds=12.0):
     }
 
 
+def learned_gain(cs, henc, ncw):
+    """Es/N0 gain at CWER 1e-3 of the committed learned offset table over the plain
+    offset decoder (beta 1), both int8 with at most 10 iterations, measured on the same
+    codewords and noise (device simulation loop, seed 100) over -1.0 .. 0.5 dB."""
+    from fecsim.decoder import Decoder, Hdec
+    from fecsim.train import cwer_curve, load_offsets, snr_at
+    tab, rec = load_offsets("11ad12_T10_int8")
+    hdec = Hdec.from_tables(cs.dec)
+    plain = Decoder(hdec, "Vec64c_min", 10, 31.0, 1.0)
+    learned = Decoder(hdec, "Vec64c_min", 10, 31.0, 1.0)
+    learned.set_offsets(tab)
+    snrs = [-1.0 + 0.125 * i for i in range(13)]
+    cp = cwer_curve(cs, henc, plain, snrs, ncw)
+    cl = cwer_curve(cs, henc, learned, snrs, ncw)
+    sp, sl = snr_at(cp), snr_at(cl)
+    return {"gain_db": (sp - sl) if sp is not None and sl is not None else None,
+            "table": "simd-fecfun_amd/fecsim/learned/11ad12_T10_int8.json", "made_by": rec["command"],
+            "es_n0_at_cwer_1e-3_plain": sp, "es_n0_at_cwer_1e-3_learned": sl, "cw_per_point": ncw,
+            "cwer_plain": [c for _, c, *_ in cp], "cwer_learned": [c for _, c, *_ in cl],
+            "es_n0_db": snrs}
+
+
 def probe(args):
     import torch
     from fecsim import codes, lib as _L
@@ -157,6 +179,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ncw", type=int, default=1_000_000, help="codewords per SNR point")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-offset CWER comparison")
+    ap.add_argument("--learned-ncw", type=int, default=2_000_000, help="codewords per SNR point of that comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel", choices=["auto", "generic", "pair"], default="auto",
                     help="decoder kernel (results are identical)")
@@ -295,6 +319,11 @@ def main():
                          "note": "algorithmic bytes; 46% of LDS-active cycles are bank conflicts (32-bank passes)"},
         "learned_offset_gain_db": None,
     }
+    if not args.no_learned:
+        print("bench: learned offsets", file=sys.stderr, flush=True)
+        lg = learned_gain(cs, henc, args.learned_ncw)
+        res["learned_offset_gain_db"] = lg["gain_db"]
+        res["learned_offsets"] = lg
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         sample = [llrs[i][:100_000].cpu().numpy() for i in range(len(SNRS))]

@@ -189,6 +189,22 @@ def config5(args):
     k = 8448
     curve = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for e, x in zip(ebn0, st)]
     e3 = snr_at(curve)
+    # learned offsets retrained for this code (committed table), on the same codewords and
+    # noise; the plain decoder stays in use when the table does not gain
+    from .train import load_offsets
+    tab, rec = load_offsets("5GNR_n25344_k8448_T20_int8")
+    ldec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
+    ldec.set_offsets(tab)
+    lsim = Sim(henc, ldec, ncw, snrs, seed=100, use_graph=True)
+    lsim.run(1)              # the same loops as the plain curve: codewords and noise are paired
+    lsim.reset()
+    lsim.run(loops)
+    lst = lsim.stats()
+    curve_l = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for e, x in zip(ebn0, lst)]
+    e3_l = snr_at(curve_l)
+    gain = (e3 - e3_l) if e3 is not None and e3_l is not None else None
+    if gain is not None and gain > 0:
+        dec = ldec
     # decode-only throughput at the Eb/N0 closest to CWER 1e-3
     pick = min(range(len(ebn0)), key=lambda i: abs(math.log10(max(curve[i][1], 1e-9)) + 3))
     bits = henc.new_bits(ncw)
@@ -212,6 +228,10 @@ def config5(args):
                                      "beta 1 cap 31, max 20 iterations, 16384 cw per batch, Eb/N0 0..1.5 dB",
             "sim_seconds": dt, "sim_cw_per_s": ncw * len(snrs) * loops / dt,
             "curve_ebn0_cwer_ber_iters": curve, "ebn0_at_cwer_1e-3": e3,
+            "learned_offsets": {"table": "simd-fecfun_amd/fecsim/learned/5GNR_n25344_k8448_T20_int8.json",
+                                "made_by": rec["command"], "ebn0_at_cwer_1e-3": e3_l, "gain_db": gain,
+                                "curve_ebn0_cwer": [[e, c] for e, c, *_ in curve_l],
+                                "decoder_used": "learned" if dec is ldec else "plain (the learned table does not gain)"},
             "decode_at_ebn0": ebn0[pick], "decode_cw_per_s": ncw / (ms * 1e-3),
             "decode_info_gbit_s": ncw * k / (ms * 1e-3) / 1e9, "decode_mean_iterations": float(it.float().mean()),
             "dtype": "int8", "data": "synthetic"}

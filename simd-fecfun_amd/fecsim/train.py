@@ -216,6 +216,16 @@ def cwer_curve(cs, henc, dec, snrs, ncw, seed=100, loops=1):
     return [(s, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for s, x in zip(snrs, st)]
 
 
+def load_offsets(name: str):
+    """A committed integer offset table, fecsim/learned/<name>.json: ([max_iter][mrow]
+    float32 array for Decoder.set_offsets, the full record with its provenance)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned", name + ".json")) as f:
+        rec = json.load(f)
+    return np.asarray(rec["beta_int"], np.float32), rec
+
+
 def snr_at(curve, target=1e-3):
     """SNR where CWER crosses `target`, log-linear interpolation between the bracketing
     points; None when the curve does not bracket it."""
