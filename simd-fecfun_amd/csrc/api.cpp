@@ -187,7 +187,7 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
   d.elem_bytes = kElemBytes[elem];
   d.params = *prm;
   std::string why;
-  rc = choose_shape(d.h, d.elem_bytes, 0, &d.shape, &why);
+  rc = choose_shape(d.h, d.elem_bytes, &d.shape, &why);
   if (rc) {
     delete dec;
     return fail(rc, "decoder shape: " + why);

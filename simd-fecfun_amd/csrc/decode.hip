@@ -1,6 +1,7 @@
-// Generic layered decoder kernel: any .dat table, any element kind, dense or the
+// Generic layered decoder kernels: any .dat table, any element kind, dense or the
 // reference's padded memory layout (see decode_core.h for the semantics and the
-// wave/LDS mapping).
+// wave/LDS mapping): codewords packed into waves (k_decode) or one codeword per
+// workgroup for large codewords (k_decode_wg), and the shape choice between them.
 #include <algorithm>
 #include <cstring>
 
@@ -141,7 +142,7 @@ static int maxw_bucket(int w) { return w <= 8 ? 8 : w <= 16 ? 16 : w <= 24 ? 24 
 
 // Shape with K rows per lane: codewords per wave reduced until a wave's slice
 // (posteriors and C2V) fits the 64 KB a block may use beside the edge tables.
-static bool fit_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, int K, DecShape* s) {
+static bool fit_shape(const HdecHost& h, int elem_bytes, int K, DecShape* s) {
   const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;
   s->tab_bytes = table_bytes(h.total_hc_ones, h.mrow);
   const int room = 65536 - s->tab_bytes;
@@ -153,7 +154,7 @@ static bool fit_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, i
   s->post_stride = (N + pad - 1) / pad * pad;
   s->c2v_stride = (E + pad - 1) / pad * pad;
   s->c2v_global = false;
-  const int per_slot = (s->post_stride + s->c2v_stride) * elem_bytes + extra_slot_bytes;
+  const int per_slot = (s->post_stride + s->c2v_stride) * elem_bytes;
   s->cw_per_wave = 64 / L;
   while (s->cw_per_wave > 1 && s->cw_per_wave * per_slot > room) --s->cw_per_wave;
   s->wave_bytes = s->cw_per_wave * per_slot;
@@ -182,12 +183,12 @@ static double lanes_per_cu(const DecShape& s) {
 
 // Best per-wave shape: rows per lane K giving the most row work in flight per CU once
 // the LDS limits are applied; ties -> fewer rows per lane.  Returns its score (0: none).
-static double best_wave_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, int maxw, DecShape* s) {
+static double best_wave_shape(const HdecHost& h, int elem_bytes, int maxw, DecShape* s) {
   double best = 0.0;
   for (int K = 1; K <= 16; ++K) {
     DecShape t;
     t.maxw = maxw;
-    if (!fit_shape(h, elem_bytes, extra_slot_bytes, K, &t)) continue;
+    if (!fit_shape(h, elem_bytes, K, &t)) continue;
     const double sc = lanes_per_cu(t);
     if (sc > best * (1.0 + 1e-9)) {
       best = sc;
@@ -231,13 +232,13 @@ static double wg_shape(const HdecHost& h, int elem_bytes, int maxw, DecShape* s)
   return (double)blocks * std::min(Z, t.wg_threads);
 }
 
-int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why) {
+int choose_shape(const HdecHost& h, int elem_bytes, DecShape* s, std::string* why) {
   s->maxw = maxw_bucket(h.max_row_weight);
   if (!s->maxw) { *why = "row weight above 32"; return FECSIM_E_UNSUPPORTED; }
   if (h.mcol * h.z > 65535) { *why = "mcol*z above 65535"; return FECSIM_E_UNSUPPORTED; }
   if (h.z > 1024) { *why = "z above 1024"; return FECSIM_E_UNSUPPORTED; }
   DecShape wave, wg;
-  const double sw = best_wave_shape(h, elem_bytes, extra_slot_bytes, s->maxw, &wave);
+  const double sw = best_wave_shape(h, elem_bytes, s->maxw, &wave);
   const double sg = wg_shape(h, elem_bytes, s->maxw, &wg);
   // a workgroup per codeword pays a block barrier per layer: take it when no wave slice
   // holds the codeword, or when it keeps clearly more rows in flight (codewords so large
@@ -258,7 +259,7 @@ int shape_with_rows(const HdecHost& h, int elem_bytes, int K, DecShape* s) {
   if (K == 0) {   // the per-wave choice of choose_shape (workgroup shapes excluded)
     s->maxw = maxw_bucket(h.max_row_weight);
     if (!s->maxw || h.mcol * h.z > 65535) return 1;
-    return best_wave_shape(h, elem_bytes, 0, s->maxw, s) > 0.0 ? 0 : 1;
+    return best_wave_shape(h, elem_bytes, s->maxw, s) > 0.0 ? 0 : 1;
   }
   s->maxw = maxw_bucket(h.max_row_weight);
   const int Z = h.z, N = h.mcol * Z, E = h.total_hc_ones * Z;

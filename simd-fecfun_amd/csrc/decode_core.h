@@ -1,5 +1,5 @@
-// Device core of the layered LDPC decoders (shared by the plain decoder kernel and
-// the fused simulation kernel).
+// Device core of the layered LDPC decoders (shared by the per-wave and per-workgroup
+// decoder kernels of decode.hip and the mixed-code queue kernel of multi.hip).
 //
 // Semantics restated from src/LDPCDec/LDPCdecode.cpp:79-201 and
 // decode_functions.cpp:11-137 (per check row r of layer l, edge j on column c_j

@@ -211,7 +211,7 @@ struct DecoderDev {
 int pair_i8_shape(const DecoderDev& d);
 void pair_layout(const HdecHost& h, PairLayout* pl);
 hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
-int choose_shape(const HdecHost& h, int elem_bytes, int extra_slot_bytes, DecShape* s, std::string* why);
+int choose_shape(const HdecHost& h, int elem_bytes, DecShape* s, std::string* why);
 void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 // blocks of the generic kernel resident on the whole device for decoder d's shape
