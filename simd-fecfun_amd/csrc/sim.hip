@@ -196,6 +196,17 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
   const uint64_t p0 = (uint64_t)(cw_base + a.cw_off) * per_cw;
   uint64_t hp = (p0 + 1 + t) * kGamma;
   const uint64_t dhp = (uint64_t)stride * kGamma;
+  // element offsets of column 2k of codeword cw_off + cw in the bit words and in the
+  // output, advanced with (cw, k, r) like hp (no 64-bit multiplies in the loop)
+  const uint64_t cwbits = (uint64_t)a.mcol * a.col_words;
+  uint64_t bidx = (a.cw_off + cw) * cwbits + (uint64_t)(2 * k) * a.col_words;
+  long long oidx = (long long)(a.cw_off + cw) * a.cw_stride + (long long)(2 * k) * a.col_stride + a.base;
+  const uint64_t d_b = (uint64_t)dcw * cwbits + (uint64_t)(2 * dk) * a.col_words;
+  const long long d_o = (long long)dcw * a.cw_stride + (long long)(2 * dk) * a.col_stride;
+  const uint64_t rk_b = 2ull * a.col_words;                                  // r wrapped: k + 1
+  const long long rk_o = 2LL * a.col_stride;
+  const uint64_t kc_b = cwbits - (uint64_t)(2 * a.hm) * a.col_words;        // k wrapped: cw + 1
+  const long long kc_o = a.cw_stride - 2LL * a.hm * a.col_stride;
   for (; t < total; t += stride) {
     float ns, nc;
     noise_pair(a.seed, hp, &ns, &nc);
@@ -209,7 +220,7 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
     for (int h = 0; h < 2; ++h) {
       const int col = 2 * (int)k + h;
       if (col >= a.mcol) break;
-      const uint32_t word = a.bits[(gcw * a.mcol + col) * a.col_words + (r >> 5)];
+      const uint32_t word = a.bits[bidx + (uint64_t)h * a.col_words + (r >> 5)];
       const uint32_t bit = (word >> (r & 31)) & 1u;
       const float orig = bit ? -1.0f : 1.0f;
       const float rx = __builtin_fmaf(h ? nc : ns, sigma, orig);
@@ -222,19 +233,25 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
         if ((a.punct[mw] >> (r & 31)) & 1u) l = 0.0f;
         if ((a.fill[mw] >> (r & 31)) & 1u) l = a.max_abs;
       }
-      out[(long long)gcw * a.cw_stride + (long long)col * a.col_stride + a.base + r] = to_elem<T>(l);
+      out[oidx + h * a.col_stride + r] = to_elem<T>(l);
     }
     hp += dhp;
     r += dr;
     k += dk;
     cw += dcw;
+    bidx += d_b;
+    oidx += d_o;
     if (r >= (uint32_t)a.z) {
       r -= a.z;
       ++k;
+      bidx += rk_b;
+      oidx += rk_o;
     }
     if (k >= (uint32_t)a.hm) {
       k -= a.hm;
       ++cw;
+      bidx += kc_b;
+      oidx += kc_o;
     }
   }
   const unsigned long long s = block_sum(err, red);
