@@ -143,6 +143,7 @@ struct TrainArgs {
   const uint32_t* edges;     // decoder edge table ((col*z) << 16 | shift)
   const int32_t* layer_w;
   int mrow, z, n, e, n_pad;  // n_pad: LDS elements per posterior array
+  int nedge, tab_bytes;      // total_hc_ones; LDS bytes of the [edges | layer_w] copy in front
   int iters;                 // T
   int waves;                 // codewords (waves) per block
   int is_double, global_p;   // element type; posteriors in global scratch instead of LDS

@@ -18,7 +18,8 @@
 // where x_j is the exclusive minimum and src_j the edge it came from.  Parameter
 // gradients are written per codeword and summed by k_train_sum (deterministic).
 //
-// One wave per codeword, lane owns rows lane, lane+64, ... of every layer.
+// One wave per codeword, lane owns rows lane, lane+64, ... of every layer.  Every block
+// first copies the code's edge words and layer weights into the front of its LDS.
 #include <hip/hip_runtime.h>
 
 #include "decode_core.h"
@@ -47,7 +48,7 @@ __device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t 
     v.P = reinterpret_cast<F*>(a.gscratch) + cw * (uint64_t)(2 * a.n);
     v.gP = v.P + a.n;
   } else {
-    v.P = reinterpret_cast<F*>(smem) + (size_t)wave * 2 * a.n_pad;
+    v.P = reinterpret_cast<F*>(smem + a.tab_bytes) + (size_t)wave * 2 * a.n_pad;
     v.gP = v.P + a.n_pad;
   }
   return v;
@@ -55,8 +56,8 @@ __device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t 
 
 // one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q
 template <typename F>
-__device__ __forceinline__ void fwd_row(const TrainArgs& a, F* P, const F* Cold, F* Cnew, F* Qs, int e0, int w,
-                                        int r, F al, F be, F cap) {
+__device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __restrict__ edges, F* P, const F* Cold,
+                                        F* Cnew, F* Qs, int e0, int w, int r, F al, F be, F cap) {
   const int Z = a.z;
   F q[kTrainMaxW];
   int vi[kTrainMaxW];
@@ -65,10 +66,7 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, F* P, const F* Cold,
 #pragma unroll
   for (int j = 0; j < kTrainMaxW; ++j) {
     if (j < w) {
-      const uint32_t ed = a.edges[e0 + j];
-      int v = (int)(ed & 0xffffu) + r;
-      v = v >= Z ? v - Z : v;
-      vi[j] = v + (int)(ed >> 16);
+      vi[j] = edge_var(edges[e0 + j], r, Z);
       const int ce = (e0 + j) * Z + r;
       q[j] = P[vi[j]] - Cold[ce];
       if (Qs) Qs[ce] = q[j];
@@ -103,6 +101,11 @@ template <typename F, bool GP>
 __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
+  tables_to_lds(tab, a.edges, a.nedge, a.layer_w, a.mrow);
+  __syncthreads();   // before any wave leaves
+  const uint32_t* edges = tab;
+  const int32_t* layer_w = reinterpret_cast<const int32_t*>(tab + a.nedge);
   const uint64_t cw = (uint64_t)blockIdx.x * a.waves + wave;
   if (cw >= a.batch) return;   // whole wave
   const TrainView<F> tv = train_view<F, GP>(a, cw, smem, wave);
@@ -124,9 +127,9 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
     F* Cnew = ckC + (uint64_t)(t + 1) * E;
     int e0 = 0;
     for (int l = 0; l < a.mrow; ++l) {
-      const int w = a.layer_w[l];
+      const int w = layer_w[l];
       const F al = alpha[t * a.mrow + l], be = beta[t * a.mrow + l];
-      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, P, Cold, Cnew, (F*)nullptr, e0, w, r, al, be, cap);
+      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, edges, P, Cold, Cnew, (F*)nullptr, e0, w, r, al, be, cap);
       e0 += w;
       wave_sync();
     }
@@ -139,6 +142,11 @@ template <typename F, bool GP>
 __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
+  tables_to_lds(tab, a.edges, a.nedge, a.layer_w, a.mrow);
+  __syncthreads();   // before any wave leaves
+  const uint32_t* edges = tab;
+  const int32_t* layer_w = reinterpret_cast<const int32_t*>(tab + a.nedge);
   const uint64_t cw = (uint64_t)blockIdx.x * a.waves + wave;
   if (cw >= a.batch) return;   // whole wave
   const TrainView<F> tv = train_view<F, GP>(a, cw, smem, wave);
@@ -164,14 +172,14 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
     wave_sync();
     int e0 = 0;
     for (int l = 0; l < Lr; ++l) {   // recompute, keeping q
-      const int w = a.layer_w[l];
+      const int w = layer_w[l];
       const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
-      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, P, Cold, (F*)nullptr, Qs, e0, w, r, al, be, cap);
+      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, edges, P, Cold, (F*)nullptr, Qs, e0, w, r, al, be, cap);
       e0 += w;
       wave_sync();
     }
     for (int l = Lr - 1; l >= 0; --l) {
-      const int w = a.layer_w[l];
+      const int w = layer_w[l];
       e0 -= w;
       const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
       F ga = F(0), gb = F(0);
@@ -184,10 +192,7 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
         for (int j = 0; j < kTrainMaxW; ++j) {
           gA[j] = F(0);
           if (j < w) {
-            const uint32_t ed = a.edges[e0 + j];
-            int v = (int)(ed & 0xffffu) + r;
-            v = v >= Z ? v - Z : v;
-            vi[j] = v + (int)(ed >> 16);
+            vi[j] = edge_var(edges[e0 + j], r, Z);
             q[j] = Qs[(e0 + j) * Z + r];
             const F x = fabs(q[j]) - be;
             const F A = x > F(0) ? x : F(0);
@@ -264,7 +269,7 @@ __global__ void __launch_bounds__(256) k_train_sum(const F* partial, uint32_t ba
 template <typename F, bool GP>
 hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
   const uint64_t blocks = (a.batch + a.waves - 1) / a.waves;
-  const size_t lds = GP ? 0 : (size_t)a.waves * 2 * a.n_pad * sizeof(F);
+  const size_t lds = (size_t)a.tab_bytes + (GP ? 0 : (size_t)a.waves * 2 * a.n_pad * sizeof(F));
   if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
   else hipLaunchKernelGGL((k_train_fwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
   hipError_t e = hipGetLastError();

@@ -13,7 +13,7 @@ struct fecsim_trainer {
   HdecHost h;
   fecsim_train_params p{};
   int elem_bytes = 4;
-  int waves = 4, global_p = 0, n_pad = 0;
+  int waves = 4, global_p = 0, n_pad = 0, tab_bytes = 0;
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
   void* ckP = nullptr;
@@ -53,6 +53,8 @@ TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
   a.n = h.mcol * h.z;
   a.e = h.total_hc_ones * h.z;
   a.n_pad = t->n_pad;
+  a.nedge = t->h.total_hc_ones;
+  a.tab_bytes = t->tab_bytes;
   a.iters = (int)t->p.iters;
   a.waves = t->waves;
   a.is_double = t->p.elem == FECSIM_F64;
@@ -85,9 +87,11 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   t->elem_bytes = p->elem == FECSIM_F64 ? 8 : 4;
   const size_t N = (size_t)h.mcol * h.z, E = (size_t)h.total_hc_ones * h.z, T = p->iters, B = p->max_batch;
   t->n_pad = (int)((N + 3) / 4 * 4);
+  t->tab_bytes = ((h.total_hc_ones + h.mrow) * 4 + 15) / 16 * 16;   // edge words + layer weights in LDS
   const size_t wave_bytes = 2 * (size_t)t->n_pad * t->elem_bytes;
-  if (wave_bytes <= 65536) {
-    t->waves = (int)(65536 / wave_bytes);
+  const size_t room = 65536 - (size_t)t->tab_bytes;
+  if (wave_bytes <= room) {
+    t->waves = (int)(room / wave_bytes);
     if (t->waves > 4) t->waves = 4;
   } else {
     t->global_p = 1;
