@@ -85,12 +85,16 @@ typedef struct {
 int fecsim_decoder_create(const fecsim_hdec *h, const char *kind, const fecsim_dec_params *params,
                           fecsim_decoder **out);
 /* Kernel selection.  AUTO picks the row-pair int8 kernel when the code shape was
- * compiled for it (even Z <= 64, int8 min-sum, 1 <= beta, 0 <= cap, dense layout) and
- * the generic kernel otherwise; both give identical results.  No reference
- * counterpart (the reference selects its decoder at compile time only). */
-enum { FECSIM_KERNEL_AUTO = 0, FECSIM_KERNEL_GENERIC = 1, FECSIM_KERNEL_PAIR = 2 };
+ * compiled for it (even Z <= 64, int8 min-sum, 1 <= beta, 0 <= cap, dense layout), the
+ * code-specialized int8 kernel when the whole code was compiled in (5G NR base graph 1
+ * at Z = 384; int8 min-sum, 1 <= beta, 0 <= cap, dense 16-byte aligned buffers, any
+ * offset table with entries 1..127), and the generic kernel otherwise; all give
+ * identical results.  No reference counterpart (the reference selects its decoder at
+ * compile time only). */
+enum { FECSIM_KERNEL_AUTO = 0, FECSIM_KERNEL_GENERIC = 1, FECSIM_KERNEL_PAIR = 2, FECSIM_KERNEL_CODE = 3 };
 int fecsim_decoder_select_kernel(fecsim_decoder *d, int kernel);
-/* kernel a dense decode_set call would run now (FECSIM_KERNEL_GENERIC or _PAIR) */
+/* kernel a dense decode_set call on aligned buffers would run now
+ * (FECSIM_KERNEL_GENERIC, _PAIR or _CODE) */
 int fecsim_decoder_kernel(const fecsim_decoder *d);
 
 /* element kind (FECSIM_I8..F64), mode (FECSIM_MIN/PROD), bytes per element */

@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libfecsim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["decode.hip", "pair_i8.hip", "sim.hip", "train.hip", "multi.hip", "api.cpp", "sim_api.cpp", "msim_api.cpp",
+SOURCES = ["decode.hip", "pair_i8.hip", "code_i8.hip", "sim.hip", "train.hip", "multi.hip", "api.cpp", "sim_api.cpp", "msim_api.cpp",
            "train_api.cpp", "multi_api.cpp"]
 
 FLAGS = [
@@ -41,10 +41,12 @@ def _compile(src: str) -> str:
 
 
 def generate() -> None:
-    """Code generation steps (shape list of the row-pair kernel, product-tree schedules)."""
+    """Code generation steps (shape list of the row-pair kernel, compiled-in codes of the
+    code-specialized kernel, product-tree schedules)."""
     sys.path.insert(0, HERE)
-    from fecsim import gen_pair_shapes, minprod_tree
+    from fecsim import gen_code_tables, gen_pair_shapes, minprod_tree
     gen_pair_shapes.emit(CSRC)
+    gen_code_tables.emit(CSRC)
     gen_pair_shapes._write(os.path.join(CSRC, "prodtree_gen.h"), minprod_tree.emit_hip_header(32))
 
 

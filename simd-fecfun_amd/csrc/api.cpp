@@ -226,6 +226,7 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
       return hip_fail(he, "C2V scratch");
     }
   }
+  d.code_tab = code_i8_index(d);
   d.pair_shape = pair_i8_shape(d);
   if (d.pair_shape >= 0) {
     pair_layout(h, &d.pair);
@@ -244,17 +245,27 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
 
 int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
-  if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_PAIR) return fail(FECSIM_E_INVAL, "unknown kernel");
+  if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_CODE) return fail(FECSIM_E_INVAL, "unknown kernel");
   if (kernel == FECSIM_KERNEL_PAIR && d->d.pair_shape < 0)
     return fail(FECSIM_E_UNSUPPORTED, "no row-pair kernel for this code / element / parameters");
+  if (kernel == FECSIM_KERNEL_CODE && d->d.code_tab < 0)
+    return fail(FECSIM_E_UNSUPPORTED, "no code-specialized kernel for this code / element / parameters");
   d->d.kernel = kernel;
   return FECSIM_OK;
 }
 
+// the specialized kernel a dense decode would run (FECSIM_KERNEL_GENERIC if none);
+// `aligned`: the buffers are 16-byte aligned (the code kernel moves 16-byte chunks)
+static int dense_kernel(const DecoderDev& d, bool aligned) {
+  if (d.kernel == FECSIM_KERNEL_GENERIC) return FECSIM_KERNEL_GENERIC;
+  if (d.pair_shape >= 0 && !d.d_beta_tab) return FECSIM_KERNEL_PAIR;
+  if (d.code_tab >= 0 && aligned && (!d.d_beta_tab || d.beta_tab_code_ok)) return FECSIM_KERNEL_CODE;
+  return FECSIM_KERNEL_GENERIC;
+}
+
 int fecsim_decoder_kernel(const fecsim_decoder* d) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
-  if (d->d.kernel == FECSIM_KERNEL_GENERIC || d->d.pair_shape < 0 || d->d.d_beta_tab) return FECSIM_KERNEL_GENERIC;
-  return FECSIM_KERNEL_PAIR;
+  return dense_kernel(d->d, true);
 }
 
 int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
@@ -268,6 +279,7 @@ int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
   const size_t n = (size_t)dv.params.max_iter * dv.h.mrow;
   std::vector<long long> tab;
   offsets_table(dv, beta, n, &tab);
+  dv.beta_tab_code_ok = dv.elem == FECSIM_I8 && code_i8_offsets_ok(tab);
   void* p = dv.d_beta_tab;
   hipError_t he = p ? hipSuccess : hipMalloc(&p, n * sizeof(long long));
   if (he == hipSuccess) he = hipMemcpy(p, tab.data(), n * sizeof(long long), hipMemcpyHostToDevice);
@@ -313,8 +325,11 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   a.iters = iters;
   a.c2v = c2v;
   a.tot_iter = tot_iter;
-  const bool pair = layout_p == 0 && d.pair_shape >= 0 && d.kernel != FECSIM_KERNEL_GENERIC && !d.d_beta_tab;
-  hipError_t he = pair ? launch_pair_i8(d, a, (hipStream_t)stream) : launch_decode(d, a, (hipStream_t)stream);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(llr_in) | reinterpret_cast<uintptr_t>(post_out)) & 15u) == 0;
+  const int kern = layout_p == 0 ? dense_kernel(d, aligned) : FECSIM_KERNEL_GENERIC;
+  hipError_t he = kern == FECSIM_KERNEL_PAIR   ? launch_pair_i8(d, a, (hipStream_t)stream)
+                  : kern == FECSIM_KERNEL_CODE ? launch_code_i8(d, a, (hipStream_t)stream)
+                                               : launch_decode(d, a, (hipStream_t)stream);
   if (he != hipSuccess) return hip_fail(he, "decode launch");
   return FECSIM_OK;
 }

@@ -207,11 +207,18 @@ struct DecoderDev {
   void* d_c2v_scratch = nullptr;  // generic kernel, c2v_global shapes only
   uint32_t max_grid = 0;          // blocks the generic kernel is launched with at most (0: no cap)
   void* d_beta_tab = nullptr;     // per-(iteration, layer) offsets: long long (int kinds) or double
+  int code_tab = -1;              // code compiled into the code-specialized int8 kernel, -1 if none
+  bool beta_tab_code_ok = false;  // the offset table suits that kernel (every entry 1..127)
 };
 
 int pair_i8_shape(const DecoderDev& d);
 void pair_layout(const HdecHost& h, PairLayout* pl);
 hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
+// code-specialized int8 kernel (one codeword per workgroup, code compiled in): the code's
+// index for decoder d, or -1; whether an offset table suits it; the launch
+int code_i8_index(const DecoderDev& d);
+bool code_i8_offsets_ok(const std::vector<long long>& tab);
+hipError_t launch_code_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 int choose_shape(const HdecHost& h, int elem_bytes, DecShape* s, std::string* why);
 void fill_core(const DecoderDev& d, DecCore* c);
 hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);

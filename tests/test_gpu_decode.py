@@ -231,6 +231,114 @@ def test_gpu_pair_kernel_selected_for_11ad(cuda):
     assert Decoder(Hdec.from_tables(odd.dec), "Vec64c_min", 10, 31.0, 1.0).kernel == L.KERNEL_GENERIC
 
 
+CODE_CASES = [
+    # (Eb/N0 dB, codewords, max_iter, beta, cap) on 5G NR base graph 1, Z = 384, rate 1/3;
+    # codewords 0 and 1 are replaced by full-range and +-saturated bytes
+    (0.2, 6, 20, 1.0, 31.0),    # most codewords need all 20 iterations
+    (0.6, 8, 20, 1.0, 31.0),    # most converge
+    (1.0, 8, 20, 2.0, 127.0),
+    (0.6, 6, 4, 3.0, 5.0),
+    (0.8, 4, 20, 127.0, 0.5),   # beta 127, cap 0: every C2V message is 0
+]
+
+
+def _bg1_llrs(ebn0, ncw, seed=7):
+    cs = codes.gen_structs("5GNR_n25344_k8448", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, ebn0 - 10 * np.log10(3.0), elem=O.I8, llr_scale=2.0, max_abs=24.0)
+    rng = np.random.default_rng(seed)
+    llr[0] = rng.integers(-128, 128, cs.n)
+    llr[1] = np.where(rng.random(cs.n) < 0.5, -128, 127)
+    return cs, llr
+
+
+@pytest.mark.parametrize("case", CODE_CASES, ids=lambda c: "ebn0%g-n%d-it%d-b%g-c%g" % c)
+def test_gpu_code_kernel_matches_oracle(case, cuda):
+    """The code-specialized int8 kernel (BG1 Z=384 compiled in, one codeword per
+    workgroup) against the oracle and the generic kernel: posteriors, C2V, iteration
+    counts and the summed iterations, bit for bit; out of place and in place."""
+    import torch
+    from fecsim import lib as L
+    ebn0, ncw, mi, beta, cap = case
+    _say("code kernel", case)
+    cs, llr = _bg1_llrs(ebn0, ncw)
+    ref = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=beta)
+    h = Hdec.from_tables(cs.dec)
+    for kern in (L.KERNEL_CODE, L.KERNEL_GENERIC):
+        d = Decoder(h, "Vec64c_min", mi, cap, beta)
+        d.select_kernel(kern)
+        assert d.kernel == kern
+        for inplace in (False, True):
+            x = torch.from_numpy(llr).cuda()
+            out = x if inplace else torch.empty_like(x)
+            iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+            c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
+            tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+            if inplace:
+                d.decode_set(x, iters=iters, c2v=c2v, tot_iter=tot)
+            else:
+                d.decode(x, out, iters=iters, c2v=c2v, tot_iter=tot)
+            torch.cuda.synchronize()
+            assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2]), (kern, inplace)
+            assert np.array_equal(out.cpu().numpy(), ref[0]), (kern, inplace)
+            assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1]), (kern, inplace)
+            assert int(tot.item()) == ref[3]
+    _say("  iterations", ref[2])
+
+
+def test_gpu_code_kernel_offsets_and_fallbacks(cuda):
+    """Per-(iteration, layer) offset tables run on the code kernel when every entry is
+    1..127 and match the oracle; a table with a 0 entry, beta 0, 16-bit elements and
+    buffers that are not 16-byte aligned fall back to the generic kernel, same results."""
+    import torch
+    from fecsim import lib as L
+    mi, ncw = 8, 6
+    cs, llr = _bg1_llrs(0.5, ncw, seed=9)
+    h = Hdec.from_tables(cs.dec)
+    assert Decoder(h, "Vec64c_min", mi, 31.0, 1.0).kernel == L.KERNEL_CODE
+    assert Decoder(h, "Vec64c_min", mi, 31.0, 0.0).kernel == L.KERNEL_GENERIC   # beta 0: abs(-128) wrap path
+    assert Decoder(h, "Vec32s_min", mi, 31.0, 1.0).kernel == L.KERNEL_GENERIC
+    rng = np.random.default_rng(5)
+    tab = rng.integers(1, 4, (mi, cs.dec.mrow)).astype(np.float32) + 0.6      # truncates to 1..3
+    want = O.decode_ideal(cs.dec, llr, max_iter=mi, beta=1.0, beta_table=tab)
+    d = Decoder(h, "Vec64c_min", mi, 31.0, 1.0)
+    d.set_offsets(tab)
+    assert d.kernel == L.KERNEL_CODE
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
+    d.decode(x, out, iters=it, c2v=c2v)
+    torch.cuda.synchronize()
+    assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
+    assert np.array_equal(out.cpu().numpy(), want[0])
+    assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want[1])
+    tab0 = tab.copy()
+    tab0[2, 5] = 0.0
+    want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, beta=1.0, beta_table=tab0)
+    d.set_offsets(tab0)
+    assert d.kernel == L.KERNEL_GENERIC
+    out0 = torch.empty_like(x)
+    d.decode(x, out0)
+    torch.cuda.synchronize()
+    assert np.array_equal(out0.cpu().numpy(), want0[0])
+    # 1-byte offset buffers: generic kernel, plain offsets
+    d.set_offsets(None)
+    ref = O.decode_ideal(cs.dec, llr, max_iter=mi, beta=1.0)
+    n = llr.size
+    buf_in = torch.zeros(n + 1, dtype=torch.int8, device="cuda")
+    buf_out = torch.zeros(n + 1, dtype=torch.int8, device="cuda")
+    xin = buf_in[1:].view(ncw, cs.n)
+    xin.copy_(x)
+    xout = buf_out[1:].view(ncw, cs.n)
+    it2 = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    d.decode(xin, xout, iters=it2)
+    torch.cuda.synchronize()
+    assert np.array_equal(xout.cpu().numpy(), ref[0])
+    assert np.array_equal(it2.cpu().numpy().astype(np.uint32), ref[2])
+
+
 @pytest.mark.parametrize("case", [
     # (matrix set, kind, max_iter, SNR dB, codewords): codewords whose C2V does not fit the
     # wave's LDS slice keep it in a global scratch; 3000 x 11n z81 double exceeds the
