@@ -18,8 +18,10 @@
 //   is one v_sad_u8; min1/min2 take a v_med3_u32 and a v_min_u32; the sign product is the
 //   XOR of the biased bytes (bit 7 set = non-negative).  With beta >= 1 the exclusive
 //   minimum follows from min1/min2 of |q| without an argmin (see the layer body).
-// * A workgroup barrier separates the layers.  The syndrome after each iteration is the
-//   XOR of each row's biased bytes, and a workgroup OR ends the loop
+// * A workgroup barrier separates two layers that share a column (a compile-time plan;
+//   base graph 1 needs 32 per iteration for its 46 layers), and reads of columns not
+//   written since the last barrier are issued before it.  The syndrome after each
+//   iteration is the XOR of each row's biased bytes, and a workgroup OR ends the loop
 //   (src/LDPCDec/LDPCdecode.cpp:172-197).
 #include "decode_core.h"
 #include "code_tables.h"
@@ -74,11 +76,64 @@ __device__ __forceinline__ LdsU8* lds_at(const uint32_t (&vb)[NB], int K) {
   return lds8(vb[K >> 15] + (uint32_t)(K & 32767));
 }
 
-// One layer of row r (edges E0 .. E0 + w - 1, compiled-in weight, columns and shifts).
+// Where the workgroup barriers go within an iteration.  Every layer writes all the
+// columns it reads, so layer l needs a barrier before it only when it reads a column
+// written since the last barrier (consecutive orthogonal layers share none: base graph
+// 1's extension rows pair up this way).  Before such a barrier a thread can already
+// issue the reads of the columns not written since the last one.
+template <class CT>
+struct CodePlan {
+  bool bar[CT::mrow];   // barrier before layer l
+  bool early[CT::ne];   // edge e read before that barrier
+};
+
+template <class CT>
+constexpr CodePlan<CT> code_plan() {
+  static_assert(CT::mcol <= 128, "column sets are two 64-bit words");
+  CodePlan<CT> p{};
+  uint64_t dirty[2] = {0, 0};   // columns written since the last barrier
+  int e0 = 0;
+  for (int l = 0; l < CT::mrow; ++l) {
+    uint64_t cols[2] = {0, 0};
+    for (int j = 0; j < CT::w[l]; ++j) {
+      const int c = CT::col[e0 + j];
+      cols[c >> 6] |= 1ull << (c & 63);
+    }
+    p.bar[l] = ((cols[0] & dirty[0]) | (cols[1] & dirty[1])) != 0;
+    for (int j = 0; j < CT::w[l]; ++j) {
+      const int c = CT::col[e0 + j];
+      p.early[e0 + j] = p.bar[l] && ((dirty[c >> 6] >> (c & 63)) & 1) == 0;
+    }
+    if (p.bar[l]) dirty[0] = dirty[1] = 0;
+    dirty[0] |= cols[0];
+    dirty[1] |= cols[1];
+    e0 += CT::w[l];
+  }
+  return p;
+}
+
+template <class CT>
+constexpr CodePlan<CT> kCodePlan = code_plan<CT>();
+
+// One layer of row r (edges E0 .. E0 + w - 1, compiled-in weight, columns and shifts),
+// with the workgroup barrier in front of it when the plan has one.
 template <class CT, int L, int E0, int NB, int NW>
 __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&c2v)[NW], uint32_t be, int cap) {
   constexpr int Z = CT::z, W = CT::w[L];
   static_assert(W >= 2, "row weight below 2");
+  int pb[W];
+  if constexpr (kCodePlan<CT>.bar[L]) {
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (kCodePlan<CT>.early[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (!kCodePlan<CT>.early[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+  }
   // V2C: q_j = sat8(L_j - R_j) (biased), |q_j|, min1/min2 of |q|, XOR of the bytes
   int qb[W];
   uint32_t u[W];
@@ -86,8 +141,7 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
 #pragma unroll
   for (int j = 0; j < W; ++j) {
     const int e = E0 + j;
-    const int pb = *lds_at(vb, edge_off<CT>(e));
-    const int q = clamp_byte(pb - c2v_byte(c2v[e >> 2], e & 3));
+    const int q = clamp_byte(pb[j] - c2v_byte(c2v[e >> 2], e & 3));
     qb[j] = q;
     const uint32_t uu = __builtin_amdgcn_sad_u8((uint32_t)q, 128u, 0u);
     u[j] = uu;
@@ -139,15 +193,15 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
   }
 }
 
-// layers L .. mrow-1 of one iteration, a workgroup barrier after each
+// layers L .. mrow-1 of one iteration, then the barrier before the syndrome
 template <class CT, int L, int E0, int NB, int NW>
 __device__ __forceinline__ void code_layers(const uint32_t (&vb)[NB], uint32_t (&c2v)[NW], const long long* bt,
                                             uint32_t beta, int cap) {
   const uint32_t be = bt ? (uint32_t)bt[L] : beta;
   code_layer<CT, L, E0>(vb, c2v, be, cap);
-  __syncthreads();
   __builtin_amdgcn_sched_barrier(0);   // layer by layer: bounded live ranges
   if constexpr (L + 1 < CT::mrow) code_layers<CT, L + 1, E0 + CT::w[L]>(vb, c2v, bt, beta, cap);
+  else __syncthreads();
 }
 
 // OR over layers L .. mrow-1 of the row's parity in bit 7 (set: check not satisfied):
