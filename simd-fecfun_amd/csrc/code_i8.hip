@@ -81,11 +81,18 @@ __device__ __forceinline__ LdsU8* lds_at(const uint32_t (&vb)[NB], int K) {
 // written since the last barrier (consecutive orthogonal layers share none: base graph
 // 1's extension rows pair up this way).  Before such a barrier a thread can already
 // issue the reads of the columns not written since the last one.
+// The reads of layer l + 1 that need no barrier after layer l (all of them when there is
+// none, else those of columns not written since the last one) are issued earlier still,
+// at the start of layer l, so that they overlap its arithmetic; only where the two
+// layers' weights leave the registers for it.
 template <class CT>
 struct CodePlan {
   bool bar[CT::mrow];   // barrier before layer l
   bool early[CT::ne];   // edge e read before that barrier
+  bool pre[CT::ne];     // edge e (of layer l >= 1) read at the start of layer l - 1
 };
+
+constexpr int kPreReadMaxW = 24;   // largest w[l-1] + w[l] with layer l's reads issued during layer l - 1
 
 template <class CT>
 constexpr CodePlan<CT> code_plan() {
@@ -104,6 +111,8 @@ constexpr CodePlan<CT> code_plan() {
       const int c = CT::col[e0 + j];
       p.early[e0 + j] = p.bar[l] && ((dirty[c >> 6] >> (c & 63)) & 1) == 0;
     }
+    for (int j = 0; j < CT::w[l]; ++j)
+      p.pre[e0 + j] = l > 0 && CT::w[l - 1] + CT::w[l] <= kPreReadMaxW && (!p.bar[l] || p.early[e0 + j]);
     if (p.bar[l]) dirty[0] = dirty[1] = 0;
     dirty[0] |= cols[0];
     dirty[1] |= cols[1];
@@ -112,27 +121,43 @@ constexpr CodePlan<CT> code_plan() {
   return p;
 }
 
+template <class CT, int L>
+constexpr int wnext() {
+  return L + 1 < CT::mrow ? CT::w[L + 1] : 1;
+}
+
 template <class CT>
 constexpr CodePlan<CT> kCodePlan = code_plan<CT>();
 
 // One layer of row r (edges E0 .. E0 + w - 1, compiled-in weight, columns and shifts),
 // with the workgroup barrier in front of it when the plan has one.
 template <class CT, int L, int E0, int NB, int NW>
-__device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&c2v)[NW], uint32_t be, int cap) {
+__device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&c2v)[NW], uint32_t be, int cap,
+                                           const int (&pin)[CT::w[L]], int (&pout)[wnext<CT, L>()]) {
   constexpr int Z = CT::z, W = CT::w[L];
+  constexpr const CodePlan<CT>& P = kCodePlan<CT>;
   static_assert(W >= 2, "row weight below 2");
   int pb[W];
-  if constexpr (kCodePlan<CT>.bar[L]) {
+#pragma unroll
+  for (int j = 0; j < W; ++j)
+    if (P.pre[E0 + j]) pb[j] = pin[j];   // read during the previous layer
+  if constexpr (P.bar[L]) {
 #pragma unroll
     for (int j = 0; j < W; ++j)
-      if (kCodePlan<CT>.early[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+      if (P.early[E0 + j] && !P.pre[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < W; ++j)
-      if (!kCodePlan<CT>.early[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+      if (!P.early[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
   } else {
 #pragma unroll
-    for (int j = 0; j < W; ++j) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+    for (int j = 0; j < W; ++j)
+      if (!P.pre[E0 + j]) pb[j] = *lds_at(vb, edge_off<CT>(E0 + j));
+  }
+  if constexpr (L + 1 < CT::mrow) {   // the next layer's reads that may go now
+#pragma unroll
+    for (int j = 0; j < CT::w[L + 1]; ++j)
+      if (P.pre[E0 + W + j]) pout[j] = *lds_at(vb, edge_off<CT>(E0 + W + j));
   }
   // V2C: q_j = sat8(L_j - R_j) (biased), |q_j|, min1/min2 of |q|, XOR of the bytes
   int qb[W];
@@ -196,11 +221,12 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
 // layers L .. mrow-1 of one iteration, then the barrier before the syndrome
 template <class CT, int L, int E0, int NB, int NW>
 __device__ __forceinline__ void code_layers(const uint32_t (&vb)[NB], uint32_t (&c2v)[NW], const long long* bt,
-                                            uint32_t beta, int cap) {
+                                            uint32_t beta, int cap, const int (&pin)[CT::w[L]]) {
   const uint32_t be = bt ? (uint32_t)bt[L] : beta;
-  code_layer<CT, L, E0>(vb, c2v, be, cap);
+  int pout[wnext<CT, L>()];
+  code_layer<CT, L, E0>(vb, c2v, be, cap, pin, pout);
   __builtin_amdgcn_sched_barrier(0);   // layer by layer: bounded live ranges
-  if constexpr (L + 1 < CT::mrow) code_layers<CT, L + 1, E0 + CT::w[L]>(vb, c2v, bt, beta, cap);
+  if constexpr (L + 1 < CT::mrow) code_layers<CT, L + 1, E0 + CT::w[L]>(vb, c2v, bt, beta, cap, pout);
   else __syncthreads();
 }
 
@@ -257,7 +283,8 @@ __global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(3)))
     uint32_t iter = 0;
     for (;;) {
       const long long* bt = a.beta_tab ? a.beta_tab + (size_t)iter * CT::mrow : nullptr;
-      code_layers<CT, 0, 0>(vb, c2v, bt, (uint32_t)a.beta, a.cap);
+      int pin0[CT::w[0]] = {};   // layer 0 reads nothing ahead
+      code_layers<CT, 0, 0>(vb, c2v, bt, (uint32_t)a.beta, a.cap, pin0);
       const uint32_t F = code_syndrome<CT, 0, 0>(vb);   // syndrome after the full iteration
       ++iter;
       const int any = __syncthreads_or((int)(F & 0x80u));
