@@ -93,6 +93,7 @@ struct CodePlan {
 };
 
 constexpr int kPreReadMaxW = 24;   // largest w[l-1] + w[l] with layer l's reads issued during layer l - 1
+constexpr int kKeepAbsMaxW = 12;   // largest row weight whose |q| values stay in registers
 
 template <class CT>
 constexpr CodePlan<CT> code_plan() {
@@ -161,7 +162,10 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
   }
   // V2C: q_j = sat8(L_j - R_j) (biased), |q_j|, min1/min2 of |q|, XOR of the bytes
   int qb[W];
-  uint32_t u[W];
+  // |q_j| is kept for the output pass in light layers and recomputed (one v_sad_u8) in
+  // heavy ones, whose registers set the kernel's peak
+  constexpr bool keep_u = W <= kKeepAbsMaxW;
+  uint32_t u[keep_u ? W : 1];
   uint32_t m1 = 255u, m2 = 255u, X = (W & 1) ? 0u : 0x80u;
 #pragma unroll
   for (int j = 0; j < W; ++j) {
@@ -169,7 +173,7 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
     const int q = clamp_byte(pb[j] - c2v_byte(c2v[e >> 2], e & 3));
     qb[j] = q;
     const uint32_t uu = __builtin_amdgcn_sad_u8((uint32_t)q, 128u, 0u);
-    u[j] = uu;
+    if constexpr (keep_u) u[j] = uu;
     X ^= (uint32_t)q;
     m2 = med3u(m1, uu, m2);
     m1 = m1 < uu ? m1 : uu;
@@ -191,7 +195,8 @@ __device__ __forceinline__ void code_layer(const uint32_t (&vb)[NB], uint32_t (&
   int rn[W];
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    const int m = med3i(mv1, sumb - (int)u[j], mv2);
+    const uint32_t uj = keep_u ? u[keep_u ? j : 0] : __builtin_amdgcn_sad_u8((uint32_t)qb[j], 128u, 0u);
+    const int m = med3i(mv1, sumb - (int)uj, mv2);
     const int neg = __builtin_amdgcn_sbfe(X ^ (uint32_t)qb[j], 7, 1);   // 0 or -1
     const int rv = (m ^ neg) - neg;
     rn[j] = rv;
@@ -244,7 +249,7 @@ __device__ __forceinline__ uint32_t code_syndrome(const uint32_t (&vb)[NB]) {
 }
 
 template <class CT>
-__global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(3))) k_code_i8(const CodeArgs a) {
+__global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(4))) k_code_i8(const CodeArgs a) {
   constexpr int Z = CT::z, Q = Z / 4, NE = CT::ne, NW = (NE + 3) / 4, MC = CT::mcol;
   constexpr int LDS = (3 * MC + 1) * Z;
   constexpr int NB = (LDS + 32767) / 32768;   // base registers, one per 32 KiB: offsets stay < 32768
