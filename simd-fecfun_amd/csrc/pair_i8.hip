@@ -17,11 +17,16 @@
 //   256.  A slot's lanes read Z/2 pairs with distinct t mod Z/2, hence distinct banks.
 // * The per-lane LDS byte address of every edge's pair stays in a VGPR for the whole
 //   decode; the C2V messages too, as int8 bytes, two edges per VGPR.
-// Values are kept scaled by 256 (an int8 value v is the int16 v << 8): the int16
-// saturating v_pk_add/sub (clamp bit) then saturates exactly at -128 << 8, and one
-// v_pk_min with 127 << 8 finishes the int8 saturation, so Q = sat8(L - R) and
-// L = sat8(Q + R) take two ops each.  |Q| is max(Q, -Q) read as unsigned 16-bit, so
-// |-128 << 8| = 0x8000 = 128 << 8 exactly; R = (S xor sign Q) ? -M : M.
+// Values are int8 in the high byte of each int16 half.  A posterior is stored as
+// (v << 8) | 0xff, Q, |Q| and C2V messages as v << 8:
+// * Q = sat8(L - R): the int16 saturating v_pk_sub (clamp bit) of (L << 8 | 0xff) and
+//   (R << 8 | f) for any low byte f is 256 (L - R) + (255 - f) when that fits int16
+//   and saturates to 32767 / -32768 otherwise, so its high byte is sat8(L - R) for
+//   every L, R and f; one AND with 0xff00ff00 leaves Q << 8.  The low byte f is free,
+//   so an even edge's C2V word is subtracted as it is, its odd partner's bytes below.
+// * L = sat8(Q + R): the saturating v_pk_add of Q << 8 and R << 8, ORed with 0x00ff00ff.
+// * |Q| is max(Q, -Q) read as unsigned 16-bit, so |-128 << 8| = 0x8000 = 128 << 8
+//   exactly; R = (S xor sign Q) ? -M : M.
 #include "decode_core.h"
 #include "pair_shapes.h"
 
@@ -41,15 +46,30 @@ __device__ __forceinline__ LdsU32* lds32(uint32_t byte_addr) { return (LdsU32*)(
 typedef __attribute__((address_space(3))) uint16_t LdsU16;
 __device__ __forceinline__ LdsU16* lds16(uint32_t byte_addr) { return (LdsU16*)(size_t)byte_addr; }
 
-// C2V message pair of edge e (row lr, row lr + Z/2) as scaled int16 halves, from the
-// packed bytes; and the store back.  e is a compile-time index in the unrolled loops.
+// C2V messages of edges 2k and 2k+1 share a VGPR as int8 bytes: edge 2k in bytes 1 and 3
+// (rows lr, lr + Z/2), edge 2k+1 in bytes 0 and 2.  c2v_get returns edge e's pair as
+// int16 halves v << 8; c2v_sub_operand the subtrahend of Q = sat8(L - R), where the
+// even edge's word serves as it is (its odd partner's bytes are the free low bytes).
+// e is a compile-time index in the unrolled loops.
 template <int N>
 __device__ __forceinline__ uint32_t c2v_get(const uint32_t (&p)[N], int e) {
-  return (e & 1) ? (p[e >> 1] & 0xff00ff00u) : __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu);
+  return (e & 1) ? __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu) : (p[e >> 1] & 0xff00ff00u);
 }
 template <int N>
+__device__ __forceinline__ uint32_t c2v_sub_operand(const uint32_t (&p)[N], int e) {
+  return (e & 1) ? __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu) : p[e >> 1];
+}
+// store edge e's new pair r (int16 halves v << 8), keeping its partner's bytes
+template <int N>
 __device__ __forceinline__ void c2v_put(uint32_t (&p)[N], int e, uint32_t r) {
-  p[e >> 1] = (e & 1) ? __builtin_amdgcn_perm(r, p[e >> 1], 0x07020500u) : __builtin_amdgcn_perm(p[e >> 1], r, 0x07030501u);
+  p[e >> 1] = (e & 1) ? __builtin_amdgcn_perm(r, p[e >> 1], 0x03070105u) : __builtin_amdgcn_perm(r, p[e >> 1], 0x07020500u);
+}
+// both edges of a word at once: r_even, r_odd as int16 halves v << 8
+__device__ __forceinline__ uint32_t c2v_pack(uint32_t r_even, uint32_t r_odd) {
+  return __builtin_amdgcn_perm(r_even, r_odd, 0x07030501u);
+}
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
 __device__ __forceinline__ short2v smax(short2v a, short2v b) { return __builtin_elementwise_max(a, b); }
@@ -118,9 +138,12 @@ k_pair_i8(const PairArgs a) {
   uint32_t c2vp[(NE + 1) / 2];
 #pragma unroll
   for (int k = 0; k < (NE + 1) / 2; ++k) c2vp[k] = 0u;
-  const short2v hi8 = short2v{127 << 8, 127 << 8};                       // scaled int8 maximum
   const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
   const short2v cap2 = short2v{(short)(a.cap << 8), (short)(a.cap << 8)};
+  // cap + beta, at most 128 (|Q| <= 128, so a larger cap never binds): min(cap, m' - beta)
+  // = min(m', capb) - beta
+  const int capb_i = (a.cap + a.beta < 128 ? a.cap + a.beta : 128) << 8;
+  const ushort2v capb2 = ushort2v{(unsigned short)capb_i, (unsigned short)capb_i};
   const unsigned long long gmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
   unsigned long long my_iters = 0;   // summed by lane 0 of each slot
 
@@ -140,7 +163,7 @@ k_pair_i8(const PairArgs a) {
           const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
           for (int c = 0; c < a.mcol; ++c) {
             const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
-            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
+            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24) | 0x00ff00ffu;
             wbase[c * a.col_dwords + pos] = p;
             wbase[c * a.col_dwords + pos + 64] = (p >> 16) | (p << 16);
           }
@@ -161,36 +184,37 @@ k_pair_i8(const PairArgs a) {
     for (int l = 0; l < SH::mrow; ++l) {
       constexpr int MAXW = SH::maxw;
       const int w = SH::w[l];
-      // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128) and then raised to beta:
-      // m' = max(m, beta) is the minimum of A'_j = max(|Q_j|, beta) = A_j + beta.  With
-      // mv = min(cap, m' - beta) the exclusive minimum is
-      //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, min(mv2, m1' + m2' - beta - |Q_j|)):
-      // at the argmin |Q_j| = m1 <= m1', so the inner term is >= m2' - beta >= mv2; at any
-      // other edge |Q_j| >= m2, so the term is <= m1' - beta when m2 >= beta, and when
-      // m2 < beta then mv1 = mv2 = 0 and every M_j is 0.  Ties included.  The inner term
-      // lies in [beta - 128, 128 - beta] (scaled), so it is exact as a wrapped int16.
+      // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128).  With beta >= 1,
+      // A_j = max(|Q_j| - beta, 0) and the capped exclusive minimum is, per row,
+      //   mv1 = min(cap, max(m1, beta) - beta),  mv2 = min(max(m2, beta), cap + beta) - beta,
+      //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, min(mv2, m1 + mv2 - |Q_j|)):
+      // at the argmin |Q_j| = m1 and the inner term is mv2 (>= mv1); at any other edge
+      // |Q_j| >= m2, so the term is <= m1 + mv2 - m2 <= m1 - beta when m2 >= beta, which
+      // is <= mv1, or below 0 = mv1 when m1 < beta; when m2 < beta, mv1 = mv2 = 0.
+      // Ties included.  The inner term lies in [m1 + mv2 - 128, mv2] (scaled by 256), so
+      // it is exact as a wrapped int16 for every input.
       short2v q[MAXW];
       ushort2v A[MAXW];
-      uint32_t S = 0;
+      uint32_t S = 0;   // XOR of the layer's Q (sign bits)
       ushort2v m1 = ushort2v{0xffff, 0xffff}, m2 = ushort2v{0xffff, 0xffff};
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
           const uint32_t x = *lds32(adr[e0 + j]);
-          const short2v qq = smin(__builtin_elementwise_sub_sat(as_s(x), as_s(c2v_get(c2vp, e0 + j))), hi8);
+          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(c2v_sub_operand(c2vp, e0 + j)));
+          const short2v qq = as_s(as_u(qs) & 0xff00ff00u);   // Q << 8, Q = sat8(L - R)
           q[j] = qq;
           const ushort2v av = as_us(as_u(smax(qq, -qq)));
           A[j] = av;
-          S ^= as_u(qq);
+          if (j & 1) S = xor3(S, as_u(q[j - 1]), as_u(qq));
+          else if (j + 1 == w) S ^= as_u(qq);
           m2 = umin(m2, umax(m1, av));
           m1 = umin(m1, av);
         }
       }
-      m1 = umax(m1, beta2);
-      m2 = umax(m2, beta2);
-      const short2v mv1 = smin(cap2, as_s(as_u(m1 - beta2)));
-      const short2v mv2 = smin(cap2, as_s(as_u(m2 - beta2)));
-      uint32_t sumb_u = as_u(m1 + m2 - beta2);
+      const short2v mv1 = smin(cap2, as_s(as_u(umax(m1, beta2) - beta2)));
+      const short2v mv2 = as_s(as_u(umin(umax(m2, beta2), capb2) - beta2));
+      uint32_t sumb_u = as_u(m1 + as_us(as_u(mv2)));
       asm volatile("" : "+v"(sumb_u));   // one subtraction per edge, not re-associated into two
       const short2v sumb = as_s(sumb_u);
       // all of the layer's results first, then the stores: the per-edge chains are
@@ -204,12 +228,13 @@ k_pair_i8(const PairArgs a) {
           const short2v m = smax(mv1, smin(mv2, sumb - as_s(as_u(A[j]))));   // argmin ? min2 : min1, capped
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
-          const short2v g = smin(__builtin_elementwise_add_sat(q[j], as_s(rn)), hi8);
+          // L = sat8(Q + R), stored as (L << 8) | 0xff
+          const uint32_t g = as_u(__builtin_elementwise_add_sat(q[j], as_s(rn))) | 0x00ff00ffu;
           const int e = e0 + j;   // compile-time after unrolling
           if ((e & 1) == 0 && j + 1 < w) pend = rn;
-          else if ((e & 1) == 1 && j > 0) c2vp[e >> 1] = __builtin_amdgcn_perm(rn, pend, 0x07030501u);   // both bytes new
+          else if ((e & 1) == 1 && j > 0) c2vp[e >> 1] = c2v_pack(pend, rn);   // both bytes new
           else c2v_put(c2vp, e, rn);
-          gu[j] = as_u(g);
+          gu[j] = g;
         }
       }
 #pragma unroll
@@ -227,19 +252,26 @@ k_pair_i8(const PairArgs a) {
     // ---- syndrome after the full iteration, LDPCdecode.cpp:172-195 ----
 #pragma unroll
     for (int e = 0; e < NE; ++e) asm volatile("" : "+v"(adr[e]));   // no reuse of the layers' unpacked addresses
-    bool fail = false;
+    // F = OR over rows of the XOR of the row's posteriors (sign bits 15 and 31 matter)
+    uint32_t F = 0;
     e0 = 0;
 #pragma unroll
     for (int l = 0; l < SH::mrow; ++l) {
       const int w = SH::w[l];
-      uint32_t par = 0;
+      uint32_t xs[SH::maxw];
 #pragma unroll
       for (int j = 0; j < SH::maxw; ++j)
-        if (j < w) par ^= *lds32(adr[e0 + j]);
-      fail |= (par & 0x80008000u) != 0u;
+        if (j < w) xs[j] = *lds32(adr[e0 + j]);
+      uint32_t par = xs[0];
+#pragma unroll
+      for (int j = 1; j + 1 < SH::maxw; j += 2)
+        if (j + 1 < w) par = xor3(par, xs[j], xs[j + 1]);
+      if (w & 1) F |= par;
+      else F = __builtin_amdgcn_bitop3_b32(F, par, xs[w - 1], 0xF6);   // F | (par ^ x)
       e0 += w;
       __builtin_amdgcn_sched_barrier(0);
     }
+    const bool fail = (F & 0x80008000u) != 0u;
     const unsigned long long fb = __ballot(real && active && fail);
     ++iter;
     const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;

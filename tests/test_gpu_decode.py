@@ -220,6 +220,35 @@ def test_gpu_pair_kernel_matches_oracle(case, cuda):
         assert int(tot.item()) == ref[3]
 
 
+@pytest.mark.parametrize("ms", ["11ad 1/2", "11n 1/2 z54", "11ad 13/16"])
+def test_gpu_pair_kernel_extreme_rows(ms, cuda):
+    """Rows whose Q values are mostly -128 with one 0 (|Q| = 128 beside |Q| = 0, the
+    widest spread the exclusive-min select sees) and cap + beta both below and above 128:
+    the row-pair kernel against the oracle, bit for bit."""
+    import torch
+    from fecsim import lib as L
+    cs = codes.gen_structs(ms, simd_size=512, elem_size=8)
+    rng = np.random.default_rng(11)
+    ncw = 400
+    llr = rng.choice(np.array([-128, 0, 127, -1, 1], dtype=np.int8), p=[0.8, 0.1, 0.04, 0.03, 0.03],
+                     size=(ncw, cs.n))
+    h = Hdec.from_tables(cs.dec)
+    for beta, cap in [(1.0, 31.0), (1.0, 127.0), (2.0, 127.0), (3.0, 5.0), (127.0, 0.5), (64.0, 100.0)]:
+        _say("extreme rows", ms, beta, cap)
+        ref = O.decode_ideal(cs.dec, llr, max_iter=6, max_row_metric=cap, beta=beta)
+        d = Decoder(h, "Vec64c_min", 6, cap, beta)
+        assert d.kernel == L.KERNEL_PAIR
+        x = torch.from_numpy(llr).cuda()
+        out = torch.empty_like(x)
+        iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
+        d.decode(x, out, iters=iters, c2v=c2v)
+        torch.cuda.synchronize()
+        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1]), (beta, cap)
+        assert np.array_equal(out.cpu().numpy(), ref[0]), (beta, cap)
+        assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2]), (beta, cap)
+
+
 def test_gpu_pair_kernel_selected_for_11ad(cuda):
     from fecsim import lib as L
     cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
