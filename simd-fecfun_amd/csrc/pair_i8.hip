@@ -187,12 +187,13 @@ k_pair_i8(const PairArgs a) {
       // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128).  With beta >= 1,
       // A_j = max(|Q_j| - beta, 0) and the capped exclusive minimum is, per row,
       //   mv1 = min(cap, max(m1, beta) - beta),  mv2 = min(max(m2, beta), cap + beta) - beta,
-      //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, min(mv2, m1 + mv2 - |Q_j|)):
-      // at the argmin |Q_j| = m1 and the inner term is mv2 (>= mv1); at any other edge
-      // |Q_j| >= m2, so the term is <= m1 + mv2 - m2 <= m1 - beta when m2 >= beta, which
-      // is <= mv1, or below 0 = mv1 when m1 < beta; when m2 < beta, mv1 = mv2 = 0.
-      // Ties included.  The inner term lies in [m1 + mv2 - 128, mv2] (scaled by 256), so
-      // it is exact as a wrapped int16 for every input.
+      //   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, m1 + mv2 - |Q_j|):
+      // at the argmin |Q_j| = m1 and the term is mv2 (>= mv1); at any other edge
+      // |Q_j| >= m2, so the term is <= m1 + mv2 - m2, which is <= m1 - beta <= mv1 when
+      // m2 >= beta and the cap does not bind, <= mv2 = mv1 = cap when it does, below
+      // 0 = mv1 when m1 < beta, and <= 0 = mv1 = mv2 when m2 < beta.  Ties included.
+      // The term lies in [m1 + mv2 - 128, mv2] (scaled by 256), so it is exact as a
+      // wrapped int16 for every input.
       short2v q[MAXW];
       ushort2v A[MAXW];
       uint32_t S = 0;   // XOR of the layer's Q (sign bits)
@@ -225,7 +226,7 @@ k_pair_i8(const PairArgs a) {
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
-          const short2v m = smax(mv1, smin(mv2, sumb - as_s(as_u(A[j]))));   // argmin ? min2 : min1, capped
+          const short2v m = smax(mv1, sumb - as_s(as_u(A[j])));   // argmin ? min2 : min1, capped
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
           // L = sat8(Q + R), stored as (L << 8) | 0xff
