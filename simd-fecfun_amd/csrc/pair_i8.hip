@@ -366,9 +366,10 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
   pl->cw_per_wave = 64 / pl->half;
   pl->col_dwords = 128;
   pl->wave_dwords = h.mcol * 128;   // wave bases stay multiples of 512 bytes
-  const int wave_bytes = pl->wave_dwords * 4;
-  pl->waves = (64 * 1024) / wave_bytes;
-  if (pl->waves > 4) pl->waves = 4;
+  // one wave per block: a wave that finishes its chunk frees its SIMD slot at once
+  // instead of holding it until the block's other waves finish (11ad 1/2: +1.3% over
+  // four waves per block)
+  pl->waves = 1;
   const int L = pl->half, Z = h.z, NE = h.total_hc_ones;
   pl->adrtab.assign((size_t)pl->cw_per_wave * L * NE, 0u);
   for (int slot = 0; slot < pl->cw_per_wave; ++slot)
