@@ -175,6 +175,104 @@ __device__ __forceinline__ void row_int(typename P::S* __restrict__ post, typena
   }
 }
 
+// ---------------------------------------------------------------------------
+// one check row, int8 min-sum with beta >= 1 and the row weight W known at compile
+// time: straight-line code, all of the row's LDS reads issued before the first use.
+// Same results as row_int: with beta >= 1, wrap8(wabs8(Q) - beta) = |Q| - beta also
+// for Q = -128, so A_j = max(|Q_j| - beta, 0) with the true |Q_j| <= 128, and
+//   mv1 = min(cap, max(m1, beta) - beta),  mv2 = min(max(m2, beta), cap + beta) - beta,
+//   M_j = (j is the argmin ? mv2 : mv1) = max(mv1, m1 + mv2 - |Q_j|)
+// (m1 <= m2 the two smallest |Q|): at the argmin the term is mv2 >= mv1; at any other
+// edge |Q_j| >= m2 makes it <= m1 + mv2 - m2, which is <= mv1 (<= m1 - beta when the
+// cap does not bind, <= cap = mv1 = mv2 when it does, < 0 = mv1 when m1 < beta, and
+// <= 0 = mv1 = mv2 when m2 < beta).  Ties included.
+// ---------------------------------------------------------------------------
+
+template <int W>
+__device__ __forceinline__ void row_i8_w(int8_t* __restrict__ post, int8_t* __restrict__ c2v,
+                                         const uint32_t* __restrict__ edges, int e0, int r, int Z, int beta,
+                                         int cap) {
+  int vi[W], lv[W], rv[W];
+  int8_t* cr = c2v + e0 * Z + r;
+#pragma unroll
+  for (int j = 0; j < W; ++j) vi[j] = edge_var(edges[e0 + j], r, Z);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    lv[j] = post[vi[j]];
+    rv[j] = cr[j * Z];
+  }
+  int q[W];
+  uint32_t u[W];
+  int S = 0;
+  uint32_t m1 = 255u, m2 = 255u;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const int d = lv[j] - rv[j];
+    const int qq = d < -128 ? -128 : (d > 127 ? 127 : d);   // sat8(L - R)
+    q[j] = qq;
+    const uint32_t a = (uint32_t)(qq < 0 ? -qq : qq);
+    u[j] = a;
+    S ^= qq;
+    const uint32_t hi = m1 > a ? m1 : a;
+    m2 = hi < m2 ? hi : m2;   // median of (m1, a, m2)
+    m1 = a < m1 ? a : m1;
+  }
+  const int b = beta;
+  const int m1p = (int)m1 > b ? (int)m1 : b, m2p = (int)m2 > b ? (int)m2 : b;
+  const int mv1 = m1p - b < cap ? m1p - b : cap;
+  const int mv2 = (m2p < cap + b ? m2p : cap + b) - b;
+  const int sumb = (int)m1 + mv2;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const int t = sumb - (int)u[j];
+    const int m = t > mv1 ? t : mv1;
+    const int neg = (S ^ q[j]) >> 31;   // 0 or -1
+    const int rn = (m ^ neg) - neg;
+    const int g = q[j] + rn;
+    post[vi[j]] = (int8_t)(g < -128 ? -128 : (g > 127 ? 127 : g));
+    cr[j * Z] = (int8_t)rn;
+  }
+}
+
+// the same check row's parity (sign bits of its int8 variables)
+template <int W>
+__device__ __forceinline__ bool row_parity_i8_w(const int8_t* __restrict__ post, const uint32_t* __restrict__ edges,
+                                                int e0, int r, int Z) {
+  int x = 0;
+#pragma unroll
+  for (int j = 0; j < W; ++j) x ^= post[edge_var(edges[e0 + j], r, Z)];
+  return x < 0;
+}
+
+// dispatch on the run-time row weight w (uniform across the wave); false when w is
+// outside 2..WMAX
+template <int W, int WMAX>
+__device__ __forceinline__ bool row_i8_sw(int w, int8_t* post, int8_t* c2v, const uint32_t* edges, int e0, int r,
+                                          int Z, int beta, int cap) {
+  if constexpr (W > WMAX) {
+    return false;
+  } else {
+    if (w == W) {
+      row_i8_w<W>(post, c2v, edges, e0, r, Z, beta, cap);
+      return true;
+    }
+    return row_i8_sw<W + 1, WMAX>(w, post, c2v, edges, e0, r, Z, beta, cap);
+  }
+}
+template <int W, int WMAX>
+__device__ __forceinline__ bool parity_i8_sw(int w, const int8_t* post, const uint32_t* edges, int e0, int r, int Z,
+                                             bool* fail) {
+  if constexpr (W > WMAX) {
+    return false;
+  } else {
+    if (w == W) {
+      *fail |= row_parity_i8_w<W>(post, edges, e0, r, Z);
+      return true;
+    }
+    return parity_i8_sw<W + 1, WMAX>(w, post, edges, e0, r, Z, fail);
+  }
+}
+
 // parity of row r of a layer (sign bits of its variables)
 template <class P, int MAXW>
 __device__ __forceinline__ bool row_parity(const typename P::S* __restrict__ post, const uint32_t* __restrict__ edges,
@@ -252,6 +350,11 @@ __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, type
 // the per-wave decode loop: every lane returns its codeword's iteration count
 // ---------------------------------------------------------------------------
 
+// the straight-line int8 rows are compiled for kernels whose row weight bound is <= 24
+// (one instance per weight 2..MAXW)
+template <class P, int MAXW>
+constexpr bool kRowI8Fast = std::is_same<P, IntPolicy<8>>::value && MAXW <= 24;
+
 // `tab`: the code's [edges | layer weights] copy in LDS
 template <class P, int MAXW>
 __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t* __restrict__ tab,
@@ -279,11 +382,19 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
           if constexpr (P::kFloat) fb = (S)reinterpret_cast<const double*>(a.beta_tab)[iter * a.mrow + l];
           else ib = reinterpret_cast<const long long*>(a.beta_tab)[iter * a.mrow + l];
         }
+        // int8 with beta >= 1 and cap >= 0: straight-line rows of compile-time weight
+        bool fast = false;
+        if constexpr (kRowI8Fast<P, MAXW>) fast = ib >= 1 && a.icap >= 0 && w >= 2 && w <= MAXW;
         for (int k = 0; k < a.rows_per_lane; ++k) {
           const int r = lr + k * L;
           if (r < Z) {
-            if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
-            else row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
+            if constexpr (P::kFloat) {
+              row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
+            } else {
+              if constexpr (kRowI8Fast<P, MAXW>)
+                if (fast && row_i8_sw<2, MAXW>(w, post, c2v, edges, e0, r, Z, (int)ib, (int)a.icap)) continue;
+              row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
+            }
           }
         }
         e0 += w;
@@ -294,7 +405,11 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
         const int w = layer_w[l];
         for (int k = 0; k < a.rows_per_lane; ++k) {
           const int r = lr + k * L;
-          if (r < Z) fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
+          if (r < Z) {
+            if constexpr (kRowI8Fast<P, MAXW>)
+              if (parity_i8_sw<2, MAXW>(w, post, edges, e0, r, Z, &fail)) continue;
+            fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
+          }
         }
         e0 += w;
       }
