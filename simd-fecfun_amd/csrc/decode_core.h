@@ -188,17 +188,44 @@ __device__ __forceinline__ void row_int(typename P::S* __restrict__ post, typena
 // <= 0 = mv1 = mv2 when m2 < beta).  Ties included.
 // ---------------------------------------------------------------------------
 
+// Edge words come from the code's table in global memory: the row's words are uniform
+// across the wave, so they load through the scalar cache into SGPRs and the per-edge
+// wrap test (s + r >= Z) is one compare of an SGPR against a lane constant.
+typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+// clamp as one v_med3_i32 (left to itself the compiler rewrites the int8 saturations
+// as 16-bit shift / saturating-subtract / shift sequences)
+__device__ __forceinline__ int med3_i32(int a, int b, int c) {
+  int d;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+__device__ __forceinline__ ConstU32* uniform_table(const uint32_t* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return (ConstU32*)(((unsigned long long)hi << 32) | lo);
+}
+
 template <int W>
 __device__ __forceinline__ void row_i8_w(int8_t* __restrict__ post, int8_t* __restrict__ c2v,
-                                         const uint32_t* __restrict__ edges, int e0, int r, int Z, int beta,
+                                         const uint32_t* __restrict__ gedges, int e0, int r, int Z, int beta,
                                          int cap) {
-  int vi[W], lv[W], rv[W];
+  int8_t* vp[W];
+  int lv[W], rv[W];
   int8_t* cr = c2v + e0 * Z + r;
-#pragma unroll
-  for (int j = 0; j < W; ++j) vi[j] = edge_var(edges[e0 + j], r, Z);
+  int8_t* const pr = post + r;         // variable (s + r) of column c: pr + c*Z + s, or pr - Z + ... on wrap
+  int8_t* const prz = post + r - Z;
+  const int zr = Z - r;
+  ConstU32* const ge = uniform_table(gedges + e0);
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    lv[j] = post[vi[j]];
+    const uint32_t ed = ge[j];
+    const int sh = (int)(ed & 0xffffu), cz = (int)(ed >> 16);
+    vp[j] = (sh >= zr ? prz : pr) + (cz + sh);
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    lv[j] = *vp[j];
     rv[j] = cr[j * Z];
   }
   int q[W];
@@ -207,8 +234,7 @@ __device__ __forceinline__ void row_i8_w(int8_t* __restrict__ post, int8_t* __re
   uint32_t m1 = 255u, m2 = 255u;
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    const int d = lv[j] - rv[j];
-    const int qq = d < -128 ? -128 : (d > 127 ? 127 : d);   // sat8(L - R)
+    const int qq = med3_i32(lv[j] - rv[j], -128, 127);   // sat8(L - R)
     q[j] = qq;
     const uint32_t a = (uint32_t)(qq < 0 ? -qq : qq);
     u[j] = a;
@@ -229,18 +255,26 @@ __device__ __forceinline__ void row_i8_w(int8_t* __restrict__ post, int8_t* __re
     const int neg = (S ^ q[j]) >> 31;   // 0 or -1
     const int rn = (m ^ neg) - neg;
     const int g = q[j] + rn;
-    post[vi[j]] = (int8_t)(g < -128 ? -128 : (g > 127 ? 127 : g));
+    *vp[j] = (int8_t)med3_i32(g, -128, 127);
     cr[j * Z] = (int8_t)rn;
   }
 }
 
 // the same check row's parity (sign bits of its int8 variables)
 template <int W>
-__device__ __forceinline__ bool row_parity_i8_w(const int8_t* __restrict__ post, const uint32_t* __restrict__ edges,
+__device__ __forceinline__ bool row_parity_i8_w(const int8_t* __restrict__ post, const uint32_t* __restrict__ gedges,
                                                 int e0, int r, int Z) {
+  const int8_t* const pr = post + r;
+  const int8_t* const prz = post + r - Z;
+  const int zr = Z - r;
+  ConstU32* const ge = uniform_table(gedges + e0);
   int x = 0;
 #pragma unroll
-  for (int j = 0; j < W; ++j) x ^= post[edge_var(edges[e0 + j], r, Z)];
+  for (int j = 0; j < W; ++j) {
+    const uint32_t ed = ge[j];
+    const int sh = (int)(ed & 0xffffu), cz = (int)(ed >> 16);
+    x ^= *((sh >= zr ? prz : pr) + (cz + sh));
+  }
   return x < 0;
 }
 
@@ -392,7 +426,7 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
               row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
             } else {
               if constexpr (kRowI8Fast<P, MAXW>)
-                if (fast && row_i8_sw<2, MAXW>(w, post, c2v, edges, e0, r, Z, (int)ib, (int)a.icap)) continue;
+                if (fast && row_i8_sw<2, MAXW>(w, post, c2v, a.edges, e0, r, Z, (int)ib, (int)a.icap)) continue;
               row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)a.icap);
             }
           }
@@ -407,7 +441,7 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
           const int r = lr + k * L;
           if (r < Z) {
             if constexpr (kRowI8Fast<P, MAXW>)
-              if (parity_i8_sw<2, MAXW>(w, post, edges, e0, r, Z, &fail)) continue;
+              if (parity_i8_sw<2, MAXW>(w, post, a.edges, e0, r, Z, &fail)) continue;
             fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
           }
         }
