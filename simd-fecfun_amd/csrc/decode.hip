@@ -102,8 +102,13 @@ __global__ void __launch_bounds__(512) k_decode_wg(const KernelArgs a) {
           else ib = reinterpret_cast<const long long*>(d.beta_tab)[iter * d.mrow + l];
         }
         for (int r = tid; r < Z; r += nt) {
-          if constexpr (P::kFloat) row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
-          else row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)d.icap);
+          if constexpr (P::kFloat) {
+            if constexpr (kRowFloatFast<P, MAXW> && !GC2V)
+              if (row_float_sw<P, 2, MAXW>(w, post, c2v, d.edges, e0, r, Z, fb, fcap)) continue;
+            row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
+          } else {
+            row_int<P, MAXW>(post, c2v, edges, e0, w, r, Z, (typename P::C)ib, (typename P::C)d.icap);
+          }
         }
         e0 += w;
         __syncthreads();
@@ -112,7 +117,16 @@ __global__ void __launch_bounds__(512) k_decode_wg(const KernelArgs a) {
       e0 = 0;
       for (int l = 0; l < d.mrow; ++l) {
         const int w = layer_w[l];
-        for (int r = tid; r < Z; r += nt) fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
+        for (int r = tid; r < Z; r += nt) {
+          if constexpr (kRowFloatFast<P, MAXW>) {
+            bool f = false;
+            if (parity_float_sw<P, 2, MAXW>(w, post, d.edges, e0, r, Z, &f)) {
+              fail |= f;
+              continue;
+            }
+          }
+          fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
+        }
         e0 += w;
       }
       ++iter;

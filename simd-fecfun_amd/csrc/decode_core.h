@@ -380,6 +380,125 @@ __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, type
   }
 }
 
+// The same row with the row weight W known at compile time: straight-line code, the
+// row's edge words through the scalar cache (as row_i8_w), tanh/atanh evaluated for
+// the W edges only.  The arithmetic and its order are row_float's, so the results are
+// identical bit for bit.
+template <class P, int W>
+__device__ __forceinline__ void row_float_w(typename P::S* __restrict__ post, typename P::S* __restrict__ c2v,
+                                           const uint32_t* __restrict__ gedges, int e0, int r, int Z,
+                                           typename P::S beta, typename P::S cap) {
+  using F = typename P::S;
+  F* vp[W];
+  F lv[W], rv[W];
+  F* cr = c2v + e0 * Z + r;
+  F* const pr = post + r;
+  F* const prz = post + r - Z;
+  const int zr = Z - r;
+  ConstU32* const ge = uniform_table(gedges + e0);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const uint32_t ed = ge[j];
+    const int sh = (int)(ed & 0xffffu), cz = (int)(ed >> 16);
+    vp[j] = (sh >= zr ? prz : pr) + (cz + sh);
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    lv[j] = *vp[j];
+    rv[j] = cr[j * Z];
+  }
+  F q[W], A[W], M[W];
+  int sg = 0;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const F qq = lv[j] - rv[j];
+    q[j] = qq;
+    const F a = fabs(qq) - beta;
+    A[j] = a > F(0) ? a : F(0);   // _mm_max_ps(a, 0)
+    sg ^= (int)sign_of(qq);
+  }
+  if constexpr (P::kProd) {
+    F t[W], o[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) t[j] = fecmath::tanh_own(A[j] * F(0.5));
+    prod_tree_dyn<F>(W, t, cap, o);
+#pragma unroll
+    for (int j = 0; j < W; ++j) M[j] = F(2) * fecmath::atanh_own(o[j]);
+  } else {
+    F m1 = F(INFINITY), m2 = F(INFINITY);
+    int im = -1;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const F a = A[j];
+      const F hi = a > m1 ? a : m1;
+      m2 = hi < m2 ? hi : m2;
+      im = a < m1 ? j : im;
+      m1 = a < m1 ? a : m1;
+    }
+    const F M1 = cap < m1 ? cap : m1, M2 = cap < m2 ? cap : m2;
+#pragma unroll
+    for (int j = 0; j < W; ++j) M[j] = (j == im) ? M2 : M1;
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const bool neg = sg ^ (int)sign_of(q[j]);
+    const F rn = neg ? -M[j] : M[j];
+    *vp[j] = q[j] + rn;
+    cr[j * Z] = rn;
+  }
+}
+
+// parity of a float row of compile-time weight (sign bits, -0.0 counts as negative)
+template <class P, int W>
+__device__ __forceinline__ bool row_parity_float_w(const typename P::S* __restrict__ post,
+                                                   const uint32_t* __restrict__ gedges, int e0, int r, int Z) {
+  using F = typename P::S;
+  const F* const pr = post + r;
+  const F* const prz = post + r - Z;
+  const int zr = Z - r;
+  ConstU32* const ge = uniform_table(gedges + e0);
+  bool par = false;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const uint32_t ed = ge[j];
+    const int sh = (int)(ed & 0xffffu), cz = (int)(ed >> 16);
+    par ^= sign_of(*((sh >= zr ? prz : pr) + (cz + sh)));
+  }
+  return par;
+}
+
+template <class P, int W, int WMAX>
+__device__ __forceinline__ bool row_float_sw(int w, typename P::S* post, typename P::S* c2v, const uint32_t* edges,
+                                             int e0, int r, int Z, typename P::S beta, typename P::S cap) {
+  if constexpr (W > WMAX) {
+    return false;
+  } else {
+    if (w == W) {
+      row_float_w<P, W>(post, c2v, edges, e0, r, Z, beta, cap);
+      return true;
+    }
+    return row_float_sw<P, W + 1, WMAX>(w, post, c2v, edges, e0, r, Z, beta, cap);
+  }
+}
+template <class P, int W, int WMAX>
+__device__ __forceinline__ bool parity_float_sw(int w, const typename P::S* post, const uint32_t* edges, int e0, int r,
+                                                int Z, bool* fail) {
+  if constexpr (W > WMAX) {
+    return false;
+  } else {
+    if (w == W) {
+      *fail |= row_parity_float_w<P, W>(post, edges, e0, r, Z);
+      return true;
+    }
+    return parity_float_sw<P, W + 1, WMAX>(w, post, edges, e0, r, Z, fail);
+  }
+}
+
+// float rows are specialized per weight for kernels whose weight bound is <= 8 (the
+// codes of the float workloads: 16e, 11n and 11ad rate 1/2, weights 5..8)
+template <class P, int MAXW>
+constexpr bool kRowFloatFast = P::kFloat && MAXW <= 8;
+
 // ---------------------------------------------------------------------------
 // the per-wave decode loop: every lane returns its codeword's iteration count
 // ---------------------------------------------------------------------------
@@ -423,6 +542,8 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
           const int r = lr + k * L;
           if (r < Z) {
             if constexpr (P::kFloat) {
+              if constexpr (kRowFloatFast<P, MAXW>)
+                if (row_float_sw<P, 2, MAXW>(w, post, c2v, a.edges, e0, r, Z, fb, fcap)) continue;
               row_float<P, MAXW>(post, c2v, edges, e0, w, r, Z, fb, fcap);
             } else {
               if constexpr (kRowI8Fast<P, MAXW>)
@@ -442,6 +563,8 @@ __device__ __forceinline__ uint32_t decode_loop(const DecCore& a, const uint32_t
           if (r < Z) {
             if constexpr (kRowI8Fast<P, MAXW>)
               if (parity_i8_sw<2, MAXW>(w, post, a.edges, e0, r, Z, &fail)) continue;
+            if constexpr (kRowFloatFast<P, MAXW>)
+              if (parity_float_sw<P, 2, MAXW>(w, post, a.edges, e0, r, Z, &fail)) continue;
             fail |= row_parity<P, MAXW>(post, edges, e0, w, r, Z);
           }
         }
