@@ -324,6 +324,14 @@ def main():
         lg = learned_gain(cs, henc, args.learned_ncw)
         res["learned_offset_gain_db"] = lg["gain_db"]
         res["learned_offsets"] = lg
+        # the same comparison on 5G NR base graph 1 (the large code of the workloads, 20
+        # iterations), whose rows range from weight 3 to 19
+        print("bench: learned offsets, 5G NR BG1", file=sys.stderr, flush=True)
+        from fecsim.benchmarks import bg1_learned_gain
+        lb = bg1_learned_gain()
+        lb.pop("_decoder", None)
+        res["learned_offset_gain_db_bg1"] = lb["gain_db"]
+        res["learned_offsets_bg1"] = lb
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         sample = [llrs[i][:100_000].cpu().numpy() for i in range(len(SNRS))]

@@ -164,12 +164,44 @@ def config4(args):
             "dtype": "f32", "data": "synthetic"}
 
 
+def bg1_learned_gain(cs=None, henc=None, plain=None):
+    """The committed BG1 offset table (fecsim/learned/5GNR_n25344_k8448_T20_int8.json)
+    against the plain int8 decoder (beta 1, cap 31, max 20 iterations), both on the same
+    codewords and noise, on a fine Eb/N0 grid around CWER 1e-3 with 2^20 codewords per
+    point: the SNR gain at CWER 1e-3 by log-linear interpolation."""
+    from . import codes
+    from .decoder import Decoder, Hdec
+    from .sim import Henc
+    from .train import cwer_curve, load_offsets, snr_at
+    if cs is None:
+        cs = codes.gen_structs("5GNR_n25344_k8448", simd_size=512, elem_size=8)
+        henc = Henc.from_tables(cs.enc)
+    if plain is None:
+        plain = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
+    rate = 10 * math.log10(8448 / 25344)
+    tab, rec = load_offsets("5GNR_n25344_k8448_T20_int8")
+    ldec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
+    ldec.set_offsets(tab)
+    fine = [0.55, 0.6, 0.65, 0.7, 0.75, 0.8]
+    fsnr = [e + rate for e in fine]
+    cp = cwer_curve(cs, henc, plain, fsnr, 262144, seed=300, loops=4)
+    cl = cwer_curve(cs, henc, ldec, fsnr, 262144, seed=300, loops=4)
+    curve_p = [(e, c) for e, (_, c, *_) in zip(fine, cp)]
+    curve_l = [(e, c) for e, (_, c, *_) in zip(fine, cl)]
+    e3_p, e3_l = snr_at(curve_p), snr_at(curve_l)
+    return {"table": "simd-fecfun_amd/fecsim/learned/5GNR_n25344_k8448_T20_int8.json", "made_by": rec["command"],
+            "code": "5G NR BG1 Z=384 (K=8448, N=25344 sent), int8, max 20 iterations", "cw_per_point": 1 << 20,
+            "ebn0_at_cwer_1e-3_plain": e3_p, "ebn0_at_cwer_1e-3_learned": e3_l,
+            "gain_db": (e3_p - e3_l) if e3_p is not None and e3_l is not None else None,
+            "curve_ebn0_cwer_plain": curve_p, "curve_ebn0_cwer_learned": curve_l, "_decoder": ldec}
+
+
 def config5(args):
     import torch
     from . import codes
     from .decoder import Decoder, Hdec
     from .sim import Henc, Sim
-    from .train import cwer_curve, snr_at
+    from .train import snr_at
     cs = codes.gen_structs("5GNR_n25344_k8448", simd_size=512, elem_size=8)
     henc = Henc.from_tables(cs.enc)
     dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
@@ -189,22 +221,11 @@ def config5(args):
     k = 8448
     curve = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for e, x in zip(ebn0, st)]
     e3 = snr_at(curve)
-    # learned offsets retrained for this code (committed table), on the same codewords and
-    # noise; the plain decoder stays in use when the table does not gain
-    from .train import load_offsets
-    tab, rec = load_offsets("5GNR_n25344_k8448_T20_int8")
-    ldec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 20, 31.0, 1.0)
-    ldec.set_offsets(tab)
-    lsim = Sim(henc, ldec, ncw, snrs, seed=100, use_graph=True)
-    lsim.run(1)              # the same loops as the plain curve: codewords and noise are paired
-    lsim.reset()
-    lsim.run(loops)
-    lst = lsim.stats()
-    curve_l = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"]) for e, x in zip(ebn0, lst)]
-    e3_l = snr_at(curve_l)
-    gain = (e3 - e3_l) if e3 is not None and e3_l is not None else None
+    lg = bg1_learned_gain(cs, henc, dec)
+    gain = lg["gain_db"]
     if gain is not None and gain > 0:
-        dec = ldec
+        dec = lg.pop("_decoder")
+    lg.pop("_decoder", None)
     # decode-only throughput at the Eb/N0 closest to CWER 1e-3
     pick = min(range(len(ebn0)), key=lambda i: abs(math.log10(max(curve[i][1], 1e-9)) + 3))
     bits = henc.new_bits(ncw)
@@ -228,10 +249,8 @@ def config5(args):
                                      "beta 1 cap 31, max 20 iterations, 16384 cw per batch, Eb/N0 0..1.5 dB",
             "sim_seconds": dt, "sim_cw_per_s": ncw * len(snrs) * loops / dt,
             "curve_ebn0_cwer_ber_iters": curve, "ebn0_at_cwer_1e-3": e3,
-            "learned_offsets": {"table": "simd-fecfun_amd/fecsim/learned/5GNR_n25344_k8448_T20_int8.json",
-                                "made_by": rec["command"], "ebn0_at_cwer_1e-3": e3_l, "gain_db": gain,
-                                "curve_ebn0_cwer": [[e, c] for e, c, *_ in curve_l],
-                                "decoder_used": "learned" if dec is ldec else "plain (the learned table does not gain)"},
+            "learned_offsets": dict(lg, decoder_used="learned" if gain is not None and gain > 0
+                                    else "plain (the learned table does not gain)"),
             "decode_at_ebn0": ebn0[pick], "decode_cw_per_s": ncw / (ms * 1e-3),
             "decode_info_gbit_s": ncw * k / (ms * 1e-3) / 1e9, "decode_mean_iterations": float(it.float().mean()),
             "dtype": "int8", "data": "synthetic"}

@@ -206,6 +206,79 @@ def refine_offsets(cs, henc, table, snr, ncw=2_000_000, sweeps=2, seed=4242, max
     return tab, first, best
 
 
+def search_offsets(cs, henc, table, snr, groups, ncw=262144, sweeps=2, seed=4242, max_iter=20, vmin=1, vmax=6,
+                   accept=0.98, log=None):
+    """Derivative-free search for the int8 decoder's offset table (LLR scale 2, |LLR| <= 24,
+    cap 31): the coordinates are (iteration group, layer) -- `groups` are iteration
+    boundaries [0, g1, ..., max_iter] and every iteration of a group shares the layer's
+    offset.  Each coordinate tries +1, -1, +2 in turn (within [vmin, vmax]) and keeps
+    the first change that leaves fewer than `accept` times the codeword errors of the
+    current table on one pre-generated batch at `snr` (paired comparison).  vmin = 1
+    keeps tables that the code-specialized kernels take.  Returns (table, errors before,
+    errors after, the same two counts on a second batch of other codewords and noise)."""
+    import torch
+    hdec = Hdec.from_tables(cs.dec)
+    dec = Decoder(hdec, "Vec64c_min", max_iter, 31.0, 1.0)
+
+    def batch(sd):
+        bits = henc.new_bits(ncw)
+        henc.gen_codewords(bits, seed=sd, cw0=0)
+        llr = torch.empty((ncw, henc.n), dtype=torch.int8, device="cuda")
+        henc.gen_llr(bits, llr, _L.I8, float(snr), seed=sd, llr_scale=2.0, max_abs_llr=24.0)
+        return bits, llr
+
+    b_fit, b_val = batch(seed), batch(seed + 1)
+    post = torch.empty_like(b_fit[1])
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+    def errors(tab, b):
+        dec.set_offsets(tab)
+        dec.decode(b[1], post)
+        cnt.zero_()
+        henc.count_errors(b[0], post, _L.I8, cnt)
+        return int(cnt[1].item())
+
+    bounds = list(groups)
+    if bounds[0] != 0 or bounds[-1] != max_iter or any(a >= b for a, b in zip(bounds, bounds[1:])):
+        raise ValueError("groups must rise from 0 to max_iter: %r" % (groups,))
+    tab = np.array(table, np.float32).copy()
+    gt = np.stack([tab[a] for a in bounds[:-1]])    # one row of offsets per group
+
+    def expand(g):
+        t = np.empty((max_iter, g.shape[1]), np.float32)
+        for i in range(len(bounds) - 1):
+            t[bounds[i]:bounds[i + 1]] = g[i]
+        return t
+
+    first = best = errors(expand(gt), b_fit)
+    val0 = errors(expand(gt), b_val)
+    if log:
+        log("search start: %d codeword errors (validation batch %d)" % (first, val0))
+    for sw in range(sweeps):
+        changed = 0
+        for g in range(gt.shape[0]):
+            if log:
+                log("search sweep %d, iterations %d..%d: %d errors" % (sw, bounds[g], bounds[g + 1] - 1, best))
+            for l in range(gt.shape[1]):
+                for d in (1.0, -1.0, 2.0):
+                    v = gt[g, l] + d
+                    if v < vmin or v > vmax:
+                        continue
+                    old = gt[g, l]
+                    gt[g, l] = v
+                    e = errors(expand(gt), b_fit)
+                    if e < best * accept:
+                        best = e
+                        changed += 1
+                        break
+                    gt[g, l] = old
+        if log:
+            log("search sweep %d: %d codeword errors (start %d), %d coordinates changed" % (sw, best, first, changed))
+        if not changed:
+            break
+    return expand(gt), first, best, val0, errors(expand(gt), b_val)
+
+
 def cwer_curve(cs, henc, dec, snrs, ncw, seed=100, loops=1):
     """CWER / BER per SNR point of the device simulation loop."""
     from .sim import Sim
@@ -290,19 +363,33 @@ def _main():
     ap.add_argument("--eval-snr", type=float, nargs=3, default=None, metavar=("LO", "HI", "STEP"),
                     help="Es/N0 grid for the CWER comparison")
     ap.add_argument("--eval-ncw", type=int, default=2_000_000)
+    ap.add_argument("--eval-loops", type=int, default=1, help="simulation loops of eval-ncw codewords per point")
     ap.add_argument("--refine-snr", type=float, default=None,
                     help="Es/N0 at which the quantized offset table is refined by coordinate descent")
     ap.add_argument("--refine-sweeps", type=int, default=2)
+    ap.add_argument("--search-snr", type=float, default=None,
+                    help="Es/N0 at which the int8 offset table is searched per (iteration group, layer)")
+    ap.add_argument("--search-groups", default=None, help="iteration boundaries, e.g. 0,1,2,4,6,10,20")
+    ap.add_argument("--search-ncw", type=int, default=262144)
+    ap.add_argument("--search-sweeps", type=int, default=2)
+    ap.add_argument("--search-from", default="trained",
+                    help="start of the search: 'trained' (the quantized trained table), 'plain' (beta = 1 "
+                         "everywhere) or the path of an earlier --out file (its integer table)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     log = lambda s: print(s, file=sys.stderr, flush=True)   # noqa: E731
     cs = codes.gen_structs(args.code, simd_size=512, elem_size=8)
     henc = Henc.from_tables(cs.enc)
     t0 = time.time()
-    alpha, beta, losses = train(cs, henc, T=args.T, batch=args.batch, steps=args.steps, ebn0=tuple(args.ebn0),
-                                lr=args.lr, init_offset=args.init_offset, init_scale=args.init_scale,
-                                learn_scale=not args.offset_only, cap=args.cap, seed=args.seed, log=log,
-                                log_every=max(1, args.steps // 20))
+    if args.steps > 0:
+        alpha, beta, losses = train(cs, henc, T=args.T, batch=args.batch, steps=args.steps, ebn0=tuple(args.ebn0),
+                                    lr=args.lr, init_offset=args.init_offset, init_scale=args.init_scale,
+                                    learn_scale=not args.offset_only, cap=args.cap, seed=args.seed, log=log,
+                                    log_every=max(1, args.steps // 20))
+    else:   # no gradient training: the plain offset 0.5 (int8 beta 1) and scale 1
+        alpha = np.ones((args.T, cs.dec.mrow), np.float32)
+        beta = np.full((args.T, cs.dec.mrow), 0.5, np.float32)
+        losses = []
     res = {"code": args.code, "T": args.T, "batch": args.batch, "steps": args.steps, "ebn0": list(args.ebn0),
            "lr": args.lr, "init_offset": args.init_offset, "init_scale": args.init_scale,
            "offset_only": args.offset_only, "cap": args.cap, "seed": args.seed, "train_seconds": time.time() - t0,
@@ -314,6 +401,20 @@ def _main():
             # integer decoder (LLR scale 2, |LLR| <= 24, cap 31): plain beta = 1 vs learned table
             q = quantize_offsets(beta, 2.0, max_iter=args.T)
             res["beta_int_rounded"] = q.tolist()
+            if args.search_snr is not None:
+                if args.search_from == "plain":
+                    start = np.ones_like(q)
+                elif args.search_from == "trained":
+                    start = np.maximum(q, 1.0)
+                else:
+                    with open(args.search_from) as f:
+                        start = np.asarray(json.load(f)["beta_int"], np.float32)
+                groups = [int(x) for x in args.search_groups.split(",")] if args.search_groups else [0, args.T]
+                q, e0, e1, v0, v1 = search_offsets(cs, henc, start, args.search_snr, groups, ncw=args.search_ncw,
+                                                   sweeps=args.search_sweeps, max_iter=args.T, log=log)
+                res.update({"search_snr": args.search_snr, "search_groups": groups, "search_from": args.search_from,
+                            "search_errors_before": e0, "search_errors_after": e1,
+                            "search_validation_errors_before": v0, "search_validation_errors_after": v1})
             if args.refine_snr is not None:
                 q, e0, e1 = refine_offsets(cs, henc, q, args.refine_snr, sweeps=args.refine_sweeps, max_iter=args.T,
                                            log=log)
@@ -323,8 +424,8 @@ def _main():
             plain = Decoder(hdec, "Vec64c_min", args.T, 31.0, 1.0)
             learned = Decoder(hdec, "Vec64c_min", args.T, 31.0, 1.0)
             learned.set_offsets(q)
-            cp = cwer_curve(cs, henc, plain, snrs, args.eval_ncw)
-            cl = cwer_curve(cs, henc, learned, snrs, args.eval_ncw)
+            cp = cwer_curve(cs, henc, plain, snrs, args.eval_ncw, loops=args.eval_loops)
+            cl = cwer_curve(cs, henc, learned, snrs, args.eval_ncw, loops=args.eval_loops)
             kind = "int8 Vec64c_min, max %d iterations with syndrome exit" % args.T
         else:
             # float unrolled decoder: plain offset 0.5 / scale 1 vs learned
@@ -333,7 +434,8 @@ def _main():
             cl = eval_float(cs, henc, alpha, beta, snrs, args.eval_ncw, args.T, cap=args.cap)
             kind = "float32 unrolled, %d iterations" % args.T
         sp, sl = snr_at(cp), snr_at(cl)
-        res.update({"eval_decoder": kind, "eval_ncw": args.eval_ncw, "curve_plain": cp, "curve_learned": cl,
+        res.update({"eval_decoder": kind, "eval_ncw": args.eval_ncw * args.eval_loops, "curve_plain": cp,
+                    "curve_learned": cl,
                     "snr_at_1e-3_plain": sp, "snr_at_1e-3_learned": sl,
                     "gain_db": (sp - sl) if sp is not None and sl is not None else None})
         log("gain at CWER 1e-3: %s dB" % res["gain_db"])
