@@ -191,3 +191,31 @@ def test_gpu_training_reduces_loss(cuda):
     al, be, losses = train(cs, henc, T=5, batch=1024, steps=60, ebn0=(2.0, 3.0), lr=2e-2, log=_say, log_every=20)
     assert np.mean(losses[-10:]) < np.mean(losses[:10])
     assert al.shape == (5, cs.dec.mrow) and be.shape == (5, cs.dec.mrow)
+
+
+def test_gpu_offset_search(cuda):
+    """search_offsets: group rows move together, values stay in 1..6, the kept table has
+    no more codeword errors than the start on the search batch, and the table it returns
+    decodes like the oracle with that table."""
+    import torch
+    from fecsim.sim import Henc
+    from fecsim.train import search_offsets
+    cs = _cs("11ad 1/2")
+    henc = Henc.from_tables(cs.enc)
+    start = np.ones((6, cs.dec.mrow), np.float32)
+    tab, e0, e1, v0, v1 = search_offsets(cs, henc, start, -1.0, [0, 2, 6], ncw=8192, sweeps=1, max_iter=6, log=_say)
+    assert tab.shape == start.shape and np.array_equal(tab, np.rint(tab))
+    assert tab.min() >= 1 and tab.max() <= 6
+    assert np.array_equal(tab[0], tab[1]) and all(np.array_equal(tab[2], tab[t]) for t in range(3, 6))
+    assert e1 <= e0 and e0 > 0 and v0 > 0
+    llr = _llrs(cs, 200, -0.5, O.I8, 2.0, 24.0)
+    want = O.decode_ideal(cs.dec, llr, max_iter=6, beta=1.0, beta_table=tab)
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 6, 31.0, 1.0)
+    d.set_offsets(tab)
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    it = torch.zeros(200, dtype=torch.int32, device="cuda")
+    d.decode(x, out, iters=it)
+    torch.cuda.synchronize()
+    assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
+    assert np.array_equal(out.cpu().numpy(), want[0])
