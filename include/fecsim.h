@@ -253,6 +253,20 @@ int fecsim_count_errors(fecsim_henc *h, uint32_t num_cw, int col_words, const ui
                         int layout_p, const void *post, unsigned long long *counts, uint32_t *cw_bit_err,
                         void *stream);
 
+/* Hard decisions (value < 0 -> 1, as LDPC_count_errors.cpp:84) of every column of
+ * num_cw decoded codewords (`post`, element `elem`, layout as fecsim_decode_set),
+ * packed like the codeword bits of fecsim_gen_codewords: bits_out[num_cw][mcol][col_words],
+ * bit r of column c at word r >> 5, bit r & 31.  The reference packs only the data
+ * columns, in place, to compare them (LDPC_count_errors.cpp:67-96). */
+int fecsim_hard_decisions(fecsim_henc *h, uint32_t num_cw, int col_words, int elem, int layout_p,
+                          const void *post, uint32_t *bits_out, void *stream);
+
+/* Histogram of per-codeword iteration counts (device iters[num_cw] as the decode calls
+ * write them): hist[min(iters[i], nbins - 1)] += 1 (device uint64[nbins], added to),
+ * nbins 1..1024.  The reference keeps only the sum (LDPCdecode.cpp:236,244). */
+int fecsim_iter_histogram(const uint32_t *iters, uint32_t num_cw, uint32_t nbins, unsigned long long *hist,
+                          void *stream);
+
 /* ------------------------------------------------------------------------- */
 /* Simulation loop                                                            */
 /* ------------------------------------------------------------------------- */
@@ -286,6 +300,11 @@ int fecsim_sim_run(fecsim_sim *s, uint32_t loops, void *stream);
 int fecsim_sim_stats_get(fecsim_sim *s, fecsim_sim_stats *out);
 /* zeroes the counters (stream-ordered); the codeword index keeps advancing */
 int fecsim_sim_reset(fecsim_sim *s, void *stream);
+/* synchronises and copies the per-SNR iteration histograms accumulated since the last
+ * reset: hist_out[num_snr][nbins] (host), bin i = codewords that stopped after i
+ * iterations; nbins must be the decoder's max_iter + 1 (at most 1024, the last bin
+ * then holds every longer count) */
+int fecsim_sim_iter_hist(fecsim_sim *s, unsigned long long *hist_out, uint32_t nbins);
 void fecsim_sim_free(fecsim_sim *s);
 
 /* Mixed-code simulation loop: a fixed queue of num_cw entries, entry i drawn uniformly

@@ -242,6 +242,9 @@ hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);
 hipError_t launch_count(int elem, const CountArgs& a, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t stream);
+hipError_t launch_hard(int elem, const CountArgs& a, uint32_t* out, hipStream_t stream);
+hipError_t launch_iter_hist(const uint32_t* iters, uint32_t n, uint32_t nbins, unsigned long long* hist,
+                            hipStream_t stream);
 hipError_t launch_advance(unsigned long long* p, unsigned long long by, hipStream_t stream);
 
 // error reporting shared by the C ABI translation units (api.cpp)

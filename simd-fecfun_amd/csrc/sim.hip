@@ -18,6 +18,8 @@
 //                transform (sin for even columns, cos for odd columns).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "decode_core.h"
 #include "fecmath.h"
 #include "internal.h"
@@ -330,6 +332,44 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   }
 }
 
+// hard decisions (value < 0 -> 1, LDPC_count_errors.cpp:84; -0.0 counts as 0 there) of
+// every column, packed like the encoder's codeword bits: one thread per output word
+template <typename T>
+__global__ void __launch_bounds__(256) k_hard(CountArgs a, uint32_t* out) {
+  const T* post = reinterpret_cast<const T*>(a.post);
+  const uint64_t per_cw = (uint64_t)a.mcol * (uint64_t)a.col_words;
+  const uint64_t words = a.num_cw * per_cw;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t cw = i / per_cw;
+    const uint32_t rest = (uint32_t)(i - cw * per_cw);
+    const uint32_t c = rest / (uint32_t)a.col_words, wd = rest - c * (uint32_t)a.col_words;
+    const T* col = post + (long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base;
+    uint32_t v = 0;
+    const int r0 = (int)wd * 32;
+    for (int b = 0; b < 32; ++b)
+      if (r0 + b < a.z) v |= (col[r0 + b] < T(0) ? 1u : 0u) << b;
+    out[i] = v;
+  }
+}
+
+// histogram of per-codeword iteration counts: block-private LDS counts, then one global
+// atomic per bin and block
+constexpr int kHistMaxBins = 1024;
+__global__ void __launch_bounds__(256) k_iter_hist(const uint32_t* iters, uint32_t n, uint32_t nbins,
+                                                   unsigned long long* hist) {
+  __shared__ uint32_t h[kHistMaxBins];
+  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t b = iters[i] < nbins - 1 ? iters[i] : nbins - 1;
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
+}
+
 // counters and the codeword offset of a simulation loop (fill and advance kernels
 // rather than memset nodes, so the loop can be captured in a graph)
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
@@ -431,6 +471,29 @@ hipError_t launch_count(int elem, const CountArgs& a0, hipStream_t st) {
   case FECSIM_F32: hipLaunchKernelGGL(k_count<float>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
   default: hipLaunchKernelGGL(k_count<double>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_hard(int elem, const CountArgs& a, uint32_t* out, hipStream_t st) {
+  if (a.num_cw == 0) return hipSuccess;
+  const uint64_t words = a.num_cw * (uint64_t)a.mcol * a.col_words;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 65536);
+  switch (elem) {
+  case FECSIM_I8: hipLaunchKernelGGL(k_hard<int8_t>, dim3(grid), dim3(256), 0, st, a, out); break;
+  case FECSIM_I16: hipLaunchKernelGGL(k_hard<int16_t>, dim3(grid), dim3(256), 0, st, a, out); break;
+  case FECSIM_I32: hipLaunchKernelGGL(k_hard<int32_t>, dim3(grid), dim3(256), 0, st, a, out); break;
+  case FECSIM_F32: hipLaunchKernelGGL(k_hard<float>, dim3(grid), dim3(256), 0, st, a, out); break;
+  default: hipLaunchKernelGGL(k_hard<double>, dim3(grid), dim3(256), 0, st, a, out); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_iter_hist(const uint32_t* iters, uint32_t n, uint32_t nbins, unsigned long long* hist,
+                            hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (nbins < 1 || nbins > (uint32_t)kHistMaxBins) return hipErrorInvalidValue;
+  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_iter_hist, dim3(grid), dim3(256), 0, st, iters, n, nbins, hist);
   return hipGetLastError();
 }
 

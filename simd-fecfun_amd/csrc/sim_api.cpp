@@ -24,6 +24,9 @@ struct fecsim_sim {
   uint32_t* bits = nullptr;                 // [num_cw][mcol][zw]
   void* llr = nullptr;                      // [num_cw][mcol*z] decoder elements
   unsigned long long* ctr = nullptr;        // [num_snr][5] + codeword offset
+  uint32_t* iters = nullptr;                // [num_cw] per-codeword iterations of the last decode
+  unsigned long long* hist = nullptr;       // [num_snr][nbins] iteration histograms
+  uint32_t nbins = 0;                       // max_iter + 1, at most 1024
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -252,8 +255,10 @@ int sim_enqueue(fecsim_sim* s, hipStream_t st) {
              c + 1, &la);
     e = launch_llr(elem, la, st);
     if (e != hipSuccess) return hip_fail(e, "llr launch");
-    rc = fecsim_decode_set(s->d, s->num_cw, 0, s->llr, nullptr, nullptr, c + 2, st);
+    rc = fecsim_decode_set(s->d, s->num_cw, 0, s->llr, s->iters, nullptr, c + 2, st);
     if (rc) return rc;
+    e = launch_iter_hist(s->iters, s->num_cw, s->nbins, s->hist + (size_t)i * s->nbins, st);
+    if (e != hipSuccess) return hip_fail(e, "histogram launch");
     CountArgs ca;
     count_args(h, s->num_cw, hh.zw, s->bits, 0, s->llr, c + 3, nullptr, c + 0, &ca);
     e = launch_count(elem, ca, st);
@@ -413,6 +418,8 @@ void fecsim_sim_free(fecsim_sim* s) {
   (void)hipFree(s->bits);
   (void)hipFree(s->llr);
   (void)hipFree(s->ctr);
+  (void)hipFree(s->iters);
+  (void)hipFree(s->hist);
   delete s;
 }
 
@@ -444,10 +451,16 @@ int fecsim_sim_create(fecsim_henc* h, fecsim_decoder* d, const fecsim_sim_params
   const size_t nbits = (size_t)p->num_cw * he.mcol * he.zw;
   const size_t nllr = (size_t)p->num_cw * hd.mcol * hd.z * d->d.elem_bytes;
   const size_t nctr = (size_t)p->num_snr * kStat + 1;
+  const uint32_t mi = d->d.params.max_iter;
+  s->nbins = mi < 1023u ? mi + 1u : 1024u;
+  const size_t nhist = (size_t)p->num_snr * s->nbins;
   hipError_t e = hipMalloc((void**)&s->bits, nbits * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s->llr, nllr);
   if (e == hipSuccess) e = hipMalloc((void**)&s->ctr, nctr * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(s->ctr, 0, nctr * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->iters, (size_t)p->num_cw * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->hist, nhist * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(s->hist, 0, nhist * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     fecsim_sim_free(s);
@@ -512,8 +525,39 @@ int fecsim_sim_stats_get(fecsim_sim* s, fecsim_sim_stats* out) {
 
 int fecsim_sim_reset(fecsim_sim* s, void* stream) {
   if (!s) return fail(FECSIM_E_INVAL, "null argument");
-  const hipError_t e = launch_fill_u64(s->ctr, (uint32_t)(s->snr.size() * kStat), 0ull, (hipStream_t)stream);
+  hipError_t e = launch_fill_u64(s->ctr, (uint32_t)(s->snr.size() * kStat), 0ull, (hipStream_t)stream);
+  if (e == hipSuccess) e = launch_fill_u64(s->hist, (uint32_t)(s->snr.size() * s->nbins), 0ull, (hipStream_t)stream);
   return e == hipSuccess ? FECSIM_OK : hip_fail(e, "reset launch");
+}
+
+int fecsim_sim_iter_hist(fecsim_sim* s, unsigned long long* hist_out, uint32_t nbins) {
+  if (!s || !hist_out) return fail(FECSIM_E_INVAL, "null argument");
+  if (nbins != s->nbins) return fail(FECSIM_E_INVAL, "nbins must be the decoder's max_iter + 1 (at most 1024)");
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(hist_out, s->hist, s->snr.size() * s->nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "histogram copy");
+}
+
+int fecsim_hard_decisions(fecsim_henc* h, uint32_t num_cw, int col_words, int elem, int layout_p, const void* post,
+                          uint32_t* bits_out, void* stream) {
+  if (!h || ((!post || !bits_out) && num_cw)) return fail(FECSIM_E_INVAL, "null argument");
+  if (col_words < h->h.zw || col_words > 4096) return fail(FECSIM_E_INVAL, "col_words below ceil(z/32)");
+  if (!elem_ok(elem)) return fail(FECSIM_E_INVAL, "unknown element kind");
+  if (layout_p < 0 || layout_p > 4096) return fail(FECSIM_E_INVAL, "layout_p outside 0..4096");
+  if (!num_cw) return FECSIM_OK;
+  CountArgs a;
+  count_args(h, num_cw, col_words, nullptr, layout_p, post, nullptr, nullptr, nullptr, &a);
+  const hipError_t e = launch_hard(elem, a, bits_out, (hipStream_t)stream);
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "hard-decision launch");
+}
+
+int fecsim_iter_histogram(const uint32_t* iters, uint32_t num_cw, uint32_t nbins, unsigned long long* hist,
+                          void* stream) {
+  if ((!iters && num_cw) || !hist) return fail(FECSIM_E_INVAL, "null argument");
+  if (nbins < 1 || nbins > 1024) return fail(FECSIM_E_INVAL, "nbins outside 1..1024");
+  const hipError_t e = launch_iter_hist(iters, num_cw, nbins, hist, (hipStream_t)stream);
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "histogram launch");
 }
 
 }  // extern "C"

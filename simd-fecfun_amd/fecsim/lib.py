@@ -43,6 +43,9 @@ SIGNATURES = [
     ("fecsim_gen_llr", _int, [_vp, _vp, _u64, _u64, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),
     ("fecsim_chan_sigma", _int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("fecsim_count_errors", _int, [_vp, _u32, _int, _vp, _int, _int, _vp, _vp, _vp, _vp]),
+    ("fecsim_hard_decisions", _int, [_vp, _u32, _int, _int, _int, _vp, _vp, _vp]),
+    ("fecsim_iter_histogram", _int, [_vp, _u32, _u32, _vp, _vp]),
+    ("fecsim_sim_iter_hist", _int, [_vp, _vp, _u32]),
     ("fecsim_sim_create", _int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     ("fecsim_sim_run", _int, [_vp, _u32, _vp]),
     ("fecsim_sim_stats_get", _int, [_vp, _vp]),

@@ -155,6 +155,24 @@ class Henc:
                                               _ptr(counts), _ptr(cw_bit_err), _stream(stream)),
                  "fecsim_count_errors")
 
+    def hard_decisions(self, post, elem: int, num_cw: int, bits_out=None, layout_p: int = 0, col_words=None,
+                       stream=None):
+        """hard decisions of every column packed like new_bits(): a uint32 CUDA tensor
+        [num_cw][mcol][col_words] (bits_out, or a new one)"""
+        import torch
+        cw = int(col_words or self.col_words)
+        if post.dtype != _torch_dtype(elem) or not post.is_cuda or not post.is_contiguous():
+            raise TypeError("post must be a contiguous CUDA tensor of the element kind")
+        if post.numel() < num_cw * _row_elems(_Info(self), layout_p):
+            raise ValueError("post holds fewer than num_cw codewords of this layout")
+        if bits_out is None:
+            bits_out = torch.empty((num_cw, self.mcol, cw), dtype=torch.int32, device=post.device)
+        if bits_out.numel() < num_cw * self.mcol * cw or not bits_out.is_cuda or not bits_out.is_contiguous():
+            raise ValueError("bits_out must be a contiguous CUDA tensor of num_cw * mcol * col_words words")
+        _L.check(_L.lib().fecsim_hard_decisions(self._h, num_cw, cw, elem, layout_p, _ptr(post), _ptr(bits_out),
+                                                _stream(stream)), "fecsim_hard_decisions")
+        return bits_out
+
     def __del__(self):
         if getattr(self, "_h", None) and _L._lib is not None:
             _L.lib().fecsim_henc_free(self._h)
@@ -194,7 +212,31 @@ class Sim:
         _L.check(_L.lib().fecsim_sim_stats_get(self._s, arr), "fecsim_sim_stats_get")
         return [{n: getattr(x, n) for n, _ in SimStats._fields_} for x in arr]
 
+    def iter_hist(self):
+        """per SNR point the histogram of iteration counts since the last reset:
+        uint64 array [num_snr][max_iter + 1], bin i = codewords that stopped after i iterations"""
+        nb = min(int(self.dec.max_iter) + 1, 1024)
+        arr = np.zeros((len(self.snr), nb), np.uint64)
+        _L.check(_L.lib().fecsim_sim_iter_hist(self._s, arr.ctypes.data, nb), "fecsim_sim_iter_hist")
+        return arr
+
     def __del__(self):
         if getattr(self, "_s", None) and _L._lib is not None:
             _L.lib().fecsim_sim_free(self._s)
             self._s = None
+
+
+def iter_histogram(iters, nbins: int, hist=None, stream=None):
+    """hist[min(iters[i], nbins - 1)] += 1 over a CUDA tensor of per-codeword iteration
+    counts (int32/uint32 as the decode calls write them); returns the int64 CUDA
+    histogram (hist, or a new zeroed one)"""
+    import torch
+    if not iters.is_cuda or iters.dtype != torch.int32 or not iters.is_contiguous():
+        raise TypeError("iters must be a contiguous int32 CUDA tensor")
+    if hist is None:
+        hist = torch.zeros(nbins, dtype=torch.int64, device=iters.device)
+    if hist.numel() < nbins or hist.dtype != torch.int64 or not hist.is_cuda:
+        raise ValueError("hist must be an int64 CUDA tensor of nbins entries")
+    _L.check(_L.lib().fecsim_iter_histogram(_ptr(iters), iters.numel(), nbins, _ptr(hist), _stream(stream)),
+             "fecsim_iter_histogram")
+    return hist

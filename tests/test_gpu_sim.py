@@ -192,7 +192,9 @@ def test_gpu_count_errors_matches_oracle(name, npar, cuda):
     assert (int(counts[1].item()), int(counts[0].item())) == (ce, be) == (1, 1)
 
 
-def _oracle_sim(cs, kind, snrs, seed, num_cw, loops, max_iter=10, mrm=31.0, beta=1.0, scale=2.0, mx=24.0):
+def _oracle_sim(cs, kind, snrs, seed, num_cw, loops, max_iter=10, mrm=31.0, beta=1.0, scale=2.0, mx=24.0, hist=None):
+    """the oracle chain's per-SNR counters; `hist` ([num_snr][max_iter + 1] uint64), when
+    given, receives the iteration histograms"""
     elem = O.ELEM_OF_CHAR[kind[kind.index("Vec") + 3:].lstrip("0123456789")[0]]
     punct, fill = _dense_masks(cs)
     st = [dict(cw=0, uncoded_bit_err=0, iters=0, bit_err=0, cw_err=0) for _ in snrs]
@@ -202,8 +204,10 @@ def _oracle_sim(cs, kind, snrs, seed, num_cw, loops, max_iter=10, mrm=31.0, beta
         for i, snr in enumerate(snrs):
             sigma, f = chan_sigma(snr, scale)
             llr, err = O.noise_llr(seed, l * num_cw, dense, cs.z, cs.enc.mcol, sigma, f, mx, punct, fill, elem)
-            post, _, _, tot = O.decode_ideal(cs.dec, llr, elem=elem, max_iter=max_iter, max_row_metric=mrm,
-                                             beta=beta, want_c2v=False)
+            post, _, its, tot = O.decode_ideal(cs.dec, llr, elem=elem, max_iter=max_iter, max_row_metric=mrm,
+                                               beta=beta, want_c2v=False)
+            if hist is not None:
+                hist[i] += np.bincount(np.asarray(its, np.int64), minlength=max_iter + 1).astype(np.uint64)
             ce, be = refsim.count_errors(cs, words, post, elem=elem)
             s = st[i]
             s["cw"] += num_cw
@@ -232,8 +236,13 @@ def test_gpu_sim_loop_matches_oracle_chain(name, kind, graph, cuda):
     torch.zeros(1000, device="cuda").sum()   # eager work between graph replays
     sim.run(loops - 1)
     got = sim.stats()
-    want = _oracle_sim(cs, kind, snrs, seed, num_cw, loops)
+    hist = np.zeros((len(snrs), 11), np.uint64)
+    want = _oracle_sim(cs, kind, snrs, seed, num_cw, loops, hist=hist)
     assert got == want, (got, want)
+    gh = sim.iter_hist()
+    assert np.array_equal(gh, hist), (gh, hist)
+    assert all(int(gh[i].sum()) == w["cw"] and int((gh[i] * np.arange(11)).sum()) == w["iters"]
+               for i, w in enumerate(want))
     # reset clears the counters; the codeword index keeps advancing (loop 3 next)
     sim.reset()
     sim.run(1)
@@ -272,3 +281,42 @@ def test_gpu_llr_per_codeword_channel_matches_oracle(cuda):
         assert np.array_equal(got[i], want[0]), i
         tot += err
     assert int(unc.item()) == tot
+
+
+@pytest.mark.parametrize("name,kind,layout_p", [("11ad 1/2", "Vec64c_min", 0), ("11ad 1/2", "Vec64c_min", 64),
+                                                ("11n 1/2 z27", "Vec16f_min", 0), ("5GNR_n600_k300", "Vec32s_min", 0)])
+def test_gpu_hard_decisions_and_iteration_histogram(name, kind, layout_p, cuda):
+    """fecsim_hard_decisions packs value < 0 of every column like the codeword bits
+    (float -0.0 -> 0); fecsim_iter_histogram equals a bincount of the iterations."""
+    import torch
+    from fecsim import lib as L
+    from fecsim.sim import iter_histogram
+    cs = _cs(name)
+    henc = Henc.from_tables(cs.enc)
+    p, ch, mode, ss, es = codes.parse_exec_name(kind)
+    elem = O.ELEM_OF_CHAR[ch]
+    ncw = 300
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    scale, mx = (1.0, 1e9) if elem == O.F32 else (2.0, 24.0)
+    llr, _ = rs.metrics(wgn, bits, 0.0, elem=elem, llr_scale=scale, max_abs=mx, P=layout_p)
+    if elem == O.F32:
+        llr.reshape(ncw, -1)[0, :8] = np.float32(-0.0)    # -0.0 is no error (LDPC_count_errors.cpp:84)
+    dec = Decoder(Hdec.from_tables(cs.dec), kind, 10, 31.0, 0.5 if elem == O.F32 else 1.0)
+    x = torch.from_numpy(np.ascontiguousarray(llr)).cuda()
+    it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    dec.decode_set(x, iters=it, layout_p=layout_p)
+    hb = henc.hard_decisions(x, elem, ncw, layout_p=layout_p)
+    h = iter_histogram(it, 11)
+    torch.cuda.synchronize()
+    post = x.cpu().numpy()
+    dense = O.ref_to_dense(post, cs.z, cs.dec.mcol, layout_p) if layout_p else post.reshape(ncw, -1)
+    hard = (dense.reshape(ncw, cs.enc.mcol, cs.z) < 0).astype(np.uint64)
+    cw = hb.shape[2]
+    want = np.zeros((ncw, cs.enc.mcol, cw), np.uint64)
+    for r in range(cs.z):
+        want[:, :, r >> 5] |= hard[:, :, r] << np.uint64(r & 31)
+    assert np.array_equal(hb.cpu().numpy().view(np.uint32).astype(np.uint64), want)
+    assert np.array_equal(h.cpu().numpy(), np.bincount(it.cpu().numpy(), minlength=11))
+    with pytest.raises(Exception):
+        iter_histogram(it, 2000)
