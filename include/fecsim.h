@@ -268,6 +268,35 @@ int fecsim_iter_histogram(const uint32_t *iters, uint32_t num_cw, uint32_t nbins
                           void *stream);
 
 /* ------------------------------------------------------------------------- */
+/* The reference's own sequential streams (replay of one of its runs)        */
+/* ------------------------------------------------------------------------- */
+
+/* State of src/RNGfunc/prng_functions.cpp: s[8] four-lane 64-bit words seeded from the
+ * 32 rand_state values (init_randstate :22-26, the values run_LDPCsim.pl passes or
+ * read_sim_params.cpp:33-43 draws): sfc64 in s[0..3] for the noise, xoshiro256++ in
+ * s[4..7] for the data bits.  The streams are sequential (normdist2 takes an extra draw
+ * of all lanes when a value is tiny), so one wave replays them: for replaying a
+ * reference run bit for bit, not a throughput path (fecsim_gen_codewords /
+ * fecsim_gen_llr are). */
+typedef struct fecsim_refrng fecsim_refrng;
+int fecsim_refrng_create(const uint64_t *rand_state /* [32] */, fecsim_refrng **out);
+void fecsim_refrng_free(fecsim_refrng *g);
+/* normdist2(wgn, simd_count) (prng_functions.cpp:148-186): simd_count * 8 floats into
+ * device wgn; simd_count even; the state advances as the reference's does */
+int fecsim_refrng_normdist2(fecsim_refrng *g, float *wgn, uint32_t simd_count, void *stream);
+/* gen_rand_databits(num_cw, Henc) (src/Top/gen_rand_databits.cpp:11-40): the data
+ * columns of bits [num_cw][mcol][col_words] (Z <= 512), masked to Z rows and to the
+ * non-filler rows; the parity columns are left to fecsim_encode */
+int fecsim_refrng_databits(fecsim_refrng *g, fecsim_henc *h, uint32_t num_cw, int col_words, uint32_t *bits,
+                           void *stream);
+/* gen_metrics (src/Top/gen_metrics.cpp:18-121) on a given noise buffer in normdist2's
+ * order (per codeword, column and group of eight rows, ceil(Z/8) groups; rows past Z
+ * are cut off and not stored); otherwise as fecsim_gen_llr */
+int fecsim_gen_llr_from_noise(fecsim_henc *h, const fecsim_chan_params *p, const float *wgn, uint32_t num_cw,
+                              int col_words, const uint32_t *bits, int elem, int layout_p, void *llr,
+                              unsigned long long *uncoded_err, void *stream);
+
+/* ------------------------------------------------------------------------- */
 /* Simulation loop                                                            */
 /* ------------------------------------------------------------------------- */
 

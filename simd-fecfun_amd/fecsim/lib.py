@@ -46,6 +46,11 @@ SIGNATURES = [
     ("fecsim_hard_decisions", _int, [_vp, _u32, _int, _int, _int, _vp, _vp, _vp]),
     ("fecsim_iter_histogram", _int, [_vp, _u32, _u32, _vp, _vp]),
     ("fecsim_sim_iter_hist", _int, [_vp, _vp, _u32]),
+    ("fecsim_refrng_create", _int, [_vp, C.POINTER(_vp)]),
+    ("fecsim_refrng_free", None, [_vp]),
+    ("fecsim_refrng_normdist2", _int, [_vp, _vp, _u32, _vp]),
+    ("fecsim_refrng_databits", _int, [_vp, _vp, _u32, _int, _vp, _vp]),
+    ("fecsim_gen_llr_from_noise", _int, [_vp, _vp, _vp, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),
     ("fecsim_sim_create", _int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     ("fecsim_sim_run", _int, [_vp, _u32, _vp]),
     ("fecsim_sim_stats_get", _int, [_vp, _vp]),

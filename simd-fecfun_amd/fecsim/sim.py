@@ -144,6 +144,21 @@ class Henc:
         _L.check(_L.lib().fecsim_gen_llr(self._h, C.byref(p), seed, cw0, num_cw, cw, _ptr(bits), elem, layout_p,
                                          _ptr(llr), _ptr(uncoded), _stream(stream)), "fecsim_gen_llr")
 
+    def gen_llr_from_noise(self, bits, wgn, llr, elem: int, snr_db: float, llr_scale: float = 2.0,
+                           max_abs_llr: float = 24.0, layout_p: int = 0, uncoded=None, col_words=None, stream=None):
+        """gen_metrics on a noise buffer in normdist2's order (RefRng.normdist2)"""
+        num_cw, cw = self._cw(bits, col_words)
+        if llr.dtype != _torch_dtype(elem) or not llr.is_cuda or not llr.is_contiguous():
+            raise TypeError("llr must be a contiguous CUDA tensor of the element kind")
+        if llr.numel() < num_cw * _row_elems(_Info(self), layout_p):
+            raise ValueError("llr holds fewer than num_cw codewords of this layout")
+        if wgn.numel() < num_cw * self.mcol * ((self.z + 7) // 8) * 8 or not wgn.is_cuda:
+            raise ValueError("wgn holds fewer values than num_cw codewords need")
+        p = ChanParams(snr_db, llr_scale, max_abs_llr)
+        _L.check(_L.lib().fecsim_gen_llr_from_noise(self._h, C.byref(p), _ptr(wgn), num_cw, cw, _ptr(bits), elem,
+                                                    layout_p, _ptr(llr), _ptr(uncoded), _stream(stream)),
+                 "fecsim_gen_llr_from_noise")
+
     def count_errors(self, bits, post, elem: int, counts, layout_p: int = 0, cw_bit_err=None, col_words=None,
                      stream=None):
         num_cw, cw = self._cw(bits, col_words)
@@ -240,3 +255,37 @@ def iter_histogram(iters, nbins: int, hist=None, stream=None):
     _L.check(_L.lib().fecsim_iter_histogram(_ptr(iters), iters.numel(), nbins, _ptr(hist), _stream(stream)),
              "fecsim_iter_histogram")
     return hist
+
+
+class RefRng:
+    """The reference's own sequential noise and data-bit streams (sfc64 / xoshiro256++
+    seeded from its 32 rand_state values), replayed on the device for bit-exact replays
+    of one of its runs; one wave, not a throughput path."""
+
+    def __init__(self, rand_state):
+        st = np.ascontiguousarray(np.asarray(rand_state, np.uint64))
+        if st.size != 32:
+            raise ValueError("rand_state holds 32 values")
+        g = C.c_void_p()
+        _L.check(_L.lib().fecsim_refrng_create(st.ctypes.data, C.byref(g)), "fecsim_refrng_create")
+        self._g = g
+
+    def normdist2(self, wgn, simd_count: int, stream=None):
+        """simd_count (even) Vec8f of normdist2 into the float32 CUDA tensor wgn"""
+        if wgn.numel() < 8 * simd_count or not wgn.is_cuda or not wgn.is_contiguous():
+            raise ValueError("wgn must be a contiguous float32 CUDA tensor of 8 * simd_count values")
+        _L.check(_L.lib().fecsim_refrng_normdist2(self._g, _ptr(wgn), simd_count, _stream(stream)),
+                 "fecsim_refrng_normdist2")
+
+    def databits(self, henc, bits, num_cw: int, col_words=None, stream=None):
+        """gen_rand_databits into the data columns of bits ([num_cw][mcol][col_words] int32)"""
+        cw = int(col_words or henc.col_words)
+        if bits.numel() < num_cw * henc.mcol * cw or not bits.is_cuda or not bits.is_contiguous():
+            raise ValueError("bits must be a contiguous CUDA tensor of num_cw * mcol * col_words words")
+        _L.check(_L.lib().fecsim_refrng_databits(self._g, henc._h, num_cw, cw, _ptr(bits), _stream(stream)),
+                 "fecsim_refrng_databits")
+
+    def __del__(self):
+        if getattr(self, "_g", None) and _L._lib is not None:
+            _L.lib().fecsim_refrng_free(self._g)
+            self._g = None

@@ -320,3 +320,53 @@ def test_gpu_hard_decisions_and_iteration_histogram(name, kind, layout_p, cuda):
     assert np.array_equal(h.cpu().numpy(), np.bincount(it.cpu().numpy(), minlength=11))
     with pytest.raises(Exception):
         iter_histogram(it, 2000)
+
+
+@pytest.mark.parametrize("name,elem,P", [("11ad 1/2", O.I8, 0), ("11n 1/2 z27", O.I8, 64), ("16e 1/2", O.F32, 0)])
+def test_gpu_reference_stream_replay(name, elem, P, cuda):
+    """The reference's sequential streams replayed on the device equal the oracle's
+    restatement of prng_functions.cpp bit for bit: normdist2 noise, gen_rand_databits +
+    encoder, gen_metrics on that noise, over two inner loops (the state carries on);
+    and a state whose first draw is tiny takes normdist2's extra draw."""
+    import torch
+    from fecsim.sim import RefRng
+    cs = _cs(name, zo=48 if name.startswith("16e") else None)
+    henc = Henc.from_tables(cs.enc)
+    seeds = refsim.perl_seeds(77)
+    rs = refsim.RefStream(cs, seeds=seeds)
+    g = RefRng(seeds[:32])
+    ncw, mcol, z = 50, cs.enc.mcol, cs.z
+    spf = (z + 7) // 8
+    scale, mx = (1.0, 1e9) if elem == O.F32 else (2.0, 24.0)
+    for loop in range(2):
+        wgn, words = rs.inner_loop(ncw)
+        count = ncw * mcol * spf
+        count += count & 1
+        gw = torch.empty(count * 8, dtype=torch.float32, device="cuda")
+        g.normdist2(gw, count)
+        bits = henc.new_bits(ncw)
+        g.databits(henc, bits, ncw)
+        henc.encode(bits)
+        torch.cuda.synchronize()
+        assert np.array_equal(gw.cpu().numpy().view(np.uint32), wgn.view(np.uint32)), loop
+        assert np.array_equal(_words(bits, mcol), words), loop
+        want, err = rs.metrics(wgn, words, -0.5, elem=elem, P=P, llr_scale=scale, max_abs=mx)
+        llr = torch.zeros(want.shape, dtype=torch.from_numpy(want).dtype, device="cuda")
+        unc = torch.zeros(1, dtype=torch.int64, device="cuda")
+        henc.gen_llr_from_noise(bits, gw, llr, elem, -0.5, llr_scale=scale, max_abs_llr=mx, layout_p=P, uncoded=unc)
+        torch.cuda.synchronize()
+        got = llr.cpu().numpy()
+        if P:
+            got, want = O.ref_to_dense(got, z, mcol, P), O.ref_to_dense(want, z, mcol, P)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), loop
+        assert int(unc.item()) == err
+    # lane 0's first sfc64 output 0x7fffffff: its 31-bit value is 0, so the extra draw
+    tiny = seeds.copy()
+    tiny[0], tiny[4], tiny[12] = 0x7FFFFFFF, 0, 0
+    rs2 = refsim.RefStream(cs, seeds=tiny)
+    w2, _ = rs2.inner_loop(4)
+    g2 = RefRng(tiny[:32])
+    gw2 = torch.empty(w2.size, dtype=torch.float32, device="cuda")
+    g2.normdist2(gw2, w2.size // 8)
+    torch.cuda.synchronize()
+    assert np.array_equal(gw2.cpu().numpy().view(np.uint32), w2.view(np.uint32))
