@@ -113,7 +113,11 @@ int main(int argc, char** argv) {
       printf("rand seed: %016lx\n", (unsigned long)sp.rand_state[i]);
     }
   }
-  if (sp.puncture_8023ca_flag) fprintf(stderr, "warning: puncture_8023ca_flag is not supported and is ignored\n");
+  // the reference reads the flag but never applies it (gen_metrics.cpp:47-49,88); a
+  // punctured 802.3ca code comes from an encoder file with a puncture list instead
+  if (sp.puncture_8023ca_flag)
+    fprintf(stderr, "warning: puncture_8023ca_flag is ignored, as in the reference; write the encoder file "
+                    "with the 802.3ca puncture list to puncture\n");
   uint64_t seed = 0;
   for (int i = 0; i < 64; ++i) seed = mix64(seed ^ sp.rand_state[i]);
 

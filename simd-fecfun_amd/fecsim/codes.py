@@ -144,6 +144,10 @@ def encode_method(matrix_set: str):
         return [1], rng(1, mrow - 1), rng(2, mrow), z
     if matrix_set.startswith("5GNR_"):
         return [1, 3], [2, 4] + rng(5, 99), [2, 4] + rng(5, 99), 0
+    if matrix_set == "802.3ca":
+        # encode_method.m:74-80; its inverse file invM2_size256 is inv(M2) over GF(2),
+        # which gen_encode solves for directly
+        return [8, 11, 12], [2, 6, 12, 1, 4, 5, 7, 11, 8], [4, 2, 3, 10, 7, 1, 9, 6, 5], 256
     raise ValueError("invalid MatrixSet argument: %r" % matrix_set)
 
 
@@ -166,21 +170,36 @@ def expand_h(hc: np.ndarray, z: int) -> np.ndarray:
 
 
 def gf2_solve_rows(a: np.ndarray, rhs: np.ndarray) -> np.ndarray:
-    """Solve A x = rhs over GF(2) for each column of rhs (A square, invertible)."""
-    n = a.shape[0]
-    aug = np.concatenate([a.astype(np.uint8) & 1, rhs.astype(np.uint8) & 1], axis=1)
+    """Solve A x = rhs over GF(2) for each column of rhs (A square, invertible).
+
+    Gauss-Jordan elimination on rows of [A | rhs] packed 64 columns to a word, so a
+    3072 x 3072 system (802.3ca's parity part) takes about a second."""
+    a = np.asarray(a)
+    rhs = np.asarray(rhs)
+    n, m = a.shape[0], rhs.shape[1]
+    nbits = n + m
+    nbytes = -(-nbits // 64) * 8
+    aug8 = np.zeros((n, nbytes), np.uint8)
+    aug8[:, :-(-nbits // 8)] = np.packbits(np.concatenate([a.astype(np.uint8) & 1, rhs.astype(np.uint8) & 1], axis=1),
+                                           axis=1, bitorder="little")
+    aug = aug8.view(np.uint64).copy()
+    one = np.uint64(1)
     for col in range(n):
-        piv = np.nonzero(aug[col:, col])[0]
+        w, b = divmod(col, 64)
+        bits = (aug[:, w] >> np.uint64(b)) & one
+        piv = np.nonzero(bits[col:])[0]
         if piv.size == 0:
             raise ValueError("matrix is singular over GF(2)")
         p = col + piv[0]
         if p != col:
             aug[[col, p]] = aug[[p, col]]
-        rows = np.nonzero(aug[:, col])[0]
+            bits[[col, p]] = bits[[p, col]]
+        rows = np.nonzero(bits)[0]
         rows = rows[rows != col]
         if rows.size:
             aug[rows] ^= aug[col]
-    return aug[:, n:]
+    out = np.unpackbits(aug.view(np.uint8), axis=1, bitorder="little", count=nbits)
+    return out[:, n:]
 
 
 # ---------------------------------------------------------------------------
@@ -395,6 +414,8 @@ def gen_encode(hc, z, num_par, direct_cols, subst_row, subst_col, punc_fill_list
     802.11n/16e, the 4-row core for 5G NR), which gives the same unique codeword.
     """
     mrow, mcol = hc.shape
+    if z > 8 * ENC_BYTES_PER_WORD:
+        raise ValueError("Z = %d does not fit the encoder's %d-bit column word" % (z, 8 * ENC_BYTES_PER_WORD))
     hlog = hc >= 0
     ndata = mcol - mrow
     par_offset, col_cnts = [], []
@@ -409,12 +430,13 @@ def gen_encode(hc, z, num_par, direct_cols, subst_row, subst_col, punc_fill_list
         h = expand_h(hc[:core, :ndata + core], z)
         m1 = h[:, :ndata * z]
         m2 = h[:, ndata * z:]
-        for dc in direct_cols:
-            # row (dc-1)*z of inv(M2)*M1: e^T inv(M2) M1 = y^T M1 with M2^T y = e
-            e = np.zeros((core * z, 1), dtype=np.uint8)
-            e[(dc - 1) * z, 0] = 1
-            y = gf2_solve_rows(m2.T.copy(), e)[:, 0]
-            row = (y.astype(np.int64) @ m1.astype(np.int64)) & 1
+        # row (dc-1)*z of inv(M2)*M1: e^T inv(M2) M1 = y^T M1 with M2^T y = e
+        e = np.zeros((core * z, len(direct_cols)), dtype=np.uint8)
+        for i, dc in enumerate(direct_cols):
+            e[(dc - 1) * z, i] = 1
+        ys = gf2_solve_rows(m2.T.copy(), e)
+        for i in range(len(direct_cols)):
+            row = np.bitwise_xor.reduce(m1[ys[:, i].astype(bool)], axis=0)
             idx = np.nonzero(row)[0]
             cols = idx // z
             par_offset += list(_wa_addr(cols, idx - cols * z))
@@ -462,10 +484,16 @@ class CodeSet:
 
 def gen_structs(matrix_set: str = "11ad 1/2", num_par: int = 1, simd_size: int = 256,
                 elem_size: int = 32, z_over: int | None = None,
-                enc_path: str | None = None, dec_path: str | None = None) -> CodeSet:
+                enc_path: str | None = None, dec_path: str | None = None,
+                puncture_8023ca: bool = False) -> CodeSet:
     """build/gen_structs.m:28-64.  simd_size/elem_size give P, the lane count the
     row relabeling is tuned for (run_LDPCsim.pl:65-70 derives them from the
-    executable name, e.g. Vec64c -> 512/8)."""
+    executable name, e.g. Vec64c -> 512/8).
+
+    puncture_8023ca: puncture the last two parity block columns of 802.3ca (their
+    LLRs are 0), as sim_struct_LDPC.h:9 describes `puncture_8023ca_flag`; the reference
+    reads that flag but its masking is commented out (gen_metrics.cpp:47-49,88), so
+    here the puncture travels in the encoder file's puncture list instead."""
     hc, z_geth, extra = get_hmatrix(matrix_set)
     direct, srow, scol, z = encode_method(matrix_set)
     if matrix_set.startswith("5GNR_"):
@@ -485,6 +513,12 @@ def gen_structs(matrix_set: str = "11ad 1/2", num_par: int = 1, simd_size: int =
         del pc
     else:
         punc_fill = [0, 0]
+    if puncture_8023ca:
+        if matrix_set != "802.3ca":
+            raise ValueError("puncture_8023ca applies to the 802.3ca matrix set only")
+        zz = (int(z_over) if z_over is not None else z * num_par)
+        mcol = hc.shape[1]
+        punc_fill = [2, 0, zz * (mcol - 2), zz * mcol - 1]   # 0-based, inclusive
     z = z * num_par
     hc = np.where(hc < 0, -1, hc * num_par)
     if z_over is not None:
@@ -514,3 +548,26 @@ def parse_exec_name(name: str):
     p, ch, mode = int(m.group(1)), m.group(2), m.group(3)
     es = EXEC_ELEM[ch]
     return p, ch, mode, p * es, es
+
+
+def _main(argv=None):
+    """Write LDPCencode.raw.dat and LDPCdecode.raw.dat, as build/gen_structs.m does."""
+    import argparse
+    import os
+    ap = argparse.ArgumentParser(description="write the simulator's LDPCencode.raw.dat / LDPCdecode.raw.dat")
+    ap.add_argument("--matrix-set", default="11ad 1/2", help="e.g. '11ad 1/2', '802.3ca', 5GNR_n25344_k8448")
+    ap.add_argument("--num-par", type=int, default=1)
+    ap.add_argument("--simd-size", type=int, default=256)
+    ap.add_argument("--elem-size", type=int, default=32)
+    ap.add_argument("--z-over", type=int, default=None)
+    ap.add_argument("--puncture-8023ca", action="store_true")
+    ap.add_argument("--out-dir", default=".")
+    a = ap.parse_args(argv)
+    cs = gen_structs(a.matrix_set, a.num_par, a.simd_size, a.elem_size, a.z_over,
+                     os.path.join(a.out_dir, "LDPCencode.raw.dat"), os.path.join(a.out_dir, "LDPCdecode.raw.dat"),
+                     puncture_8023ca=a.puncture_8023ca)
+    print("%s: Z=%d, %dx%d blocks, n=%d, k=%d" % (a.matrix_set, cs.z, cs.hc.shape[0], cs.hc.shape[1], cs.n, cs.k))
+
+
+if __name__ == "__main__":
+    _main()

@@ -29,6 +29,8 @@ ENC_CASES = [
     ("5GNR_n600_k300", 1, None),
     ("5GNR_n500_k220", 1, None),
     ("5GNR_n25344_k8448", 1, None),
+    ("802.3ca", 1, None),
+    ("802.3ca", 2, None),
 ]
 
 
@@ -250,6 +252,30 @@ def test_gpu_sim_loop_matches_oracle_chain(name, kind, graph, cuda):
     full = _oracle_sim(cs, kind, snrs, seed, num_cw, loops + 1)
     for g, w4, w3 in zip(got, full, want):
         assert g == {k: w4[k] - w3[k] for k in g}
+
+
+@pytest.mark.parametrize("kind,punct", [("Vec64c_min", False), ("Vec64c_min", True), ("Vec16f_min", True)])
+def test_gpu_8023ca_sim_loop_matches_oracle_chain(kind, punct, cuda):
+    """802.3ca (Z = 256, 12 x 69 blocks, rows of weight 22-23), with and without the
+    puncture of its last two parity block columns: encode, LLR, decode and count on
+    the device equal the oracle chain's counters and iteration histograms."""
+    import torch
+    cs = codes.gen_structs("802.3ca", simd_size=512, elem_size=8, puncture_8023ca=punct)
+    henc = Henc.from_tables(cs.enc)
+    dec = Decoder(Hdec.from_tables(cs.dec), kind, 10, 31.0, 1.0)
+    snrs = [2.0, 2.25, 3.0, 5.0]   # Es/N0 dB: the waterfall lies near 2.2 dB, 2.8 dB when punctured
+    num_cw, loops, seed = 48, 2, 77
+    _say("sim 802.3ca", kind, "punctured", punct)
+    sim = Sim(henc, dec, num_cw, snrs, seed=seed, use_graph=True)
+    sim.run(loops)
+    torch.cuda.synchronize()
+    got = sim.stats()
+    hist = np.zeros((len(snrs), 11), np.uint64)
+    want = _oracle_sim(cs, kind, snrs, seed, num_cw, loops, hist=hist)
+    assert got == want, (got, want)
+    assert np.array_equal(sim.iter_hist(), hist)
+    # the points span the waterfall: errors at the lowest SNR, none at the highest
+    assert want[0]["cw_err"] > 0 and want[-1]["cw_err"] == 0, want
 
 
 def test_gpu_llr_per_codeword_channel_matches_oracle(cuda):
