@@ -17,14 +17,15 @@
 //   256.  A slot's lanes read Z/2 pairs with distinct t mod Z/2, hence distinct banks.
 // * The per-lane LDS byte address of every edge's pair stays in a VGPR for the whole
 //   decode; the C2V messages too, as int8 bytes, two edges per VGPR.
-// Values are int8 in the high byte of each int16 half.  A posterior is stored as
-// (v << 8) | 0xff, Q, |Q| and C2V messages as v << 8:
-// * Q = sat8(L - R): the int16 saturating v_pk_sub (clamp bit) of (L << 8 | 0xff) and
-//   (R << 8 | f) for any low byte f is 256 (L - R) + (255 - f) when that fits int16
-//   and saturates to 32767 / -32768 otherwise, so its high byte is sat8(L - R) for
-//   every L, R and f; one AND with 0xff00ff00 leaves Q << 8.  The low byte f is free,
-//   so an even edge's C2V word is subtracted as it is, its odd partner's bytes below.
-// * L = sat8(Q + R): the saturating v_pk_add of Q << 8 and R << 8, ORed with 0x00ff00ff.
+// Values are int8 in the high byte of each int16 half.  Q, |Q| and C2V messages are
+// v << 8; a posterior is (v << 8) | l with l = 0, or l = 0xff where a saturating add
+// stopped at 32767 (v = 127):
+// * Q = sat8(L - R): the int16 saturating v_pk_sub (clamp bit) of (L << 8 | l) and
+//   R << 8 is 256 (L - R) + l when L - R lies in [-128, 127] (it fits int16) and
+//   saturates to 32767 / -32768 otherwise, so its high byte is sat8(L - R); one AND
+//   with 0xff00ff00 leaves Q << 8.
+// * L = sat8(Q + R): the saturating v_pk_add of Q << 8 and R << 8, whose high byte is
+//   sat8(Q + R) and whose low byte is 0 or, at 32767, 0xff.
 // * |Q| is max(Q, -Q) read as unsigned 16-bit, so |-128 << 8| = 0x8000 = 128 << 8
 //   exactly; R = (S xor sign Q) ? -M : M.
 #include "decode_core.h"
@@ -48,16 +49,10 @@ __device__ __forceinline__ LdsU16* lds16(uint32_t byte_addr) { return (LdsU16*)(
 
 // C2V messages of edges 2k and 2k+1 share a VGPR as int8 bytes: edge 2k in bytes 1 and 3
 // (rows lr, lr + Z/2), edge 2k+1 in bytes 0 and 2.  c2v_get returns edge e's pair as
-// int16 halves v << 8; c2v_sub_operand the subtrahend of Q = sat8(L - R), where the
-// even edge's word serves as it is (its odd partner's bytes are the free low bytes).
-// e is a compile-time index in the unrolled loops.
+// int16 halves v << 8 (low bytes 0).  e is a compile-time index in the unrolled loops.
 template <int N>
 __device__ __forceinline__ uint32_t c2v_get(const uint32_t (&p)[N], int e) {
   return (e & 1) ? __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu) : (p[e >> 1] & 0xff00ff00u);
-}
-template <int N>
-__device__ __forceinline__ uint32_t c2v_sub_operand(const uint32_t (&p)[N], int e) {
-  return (e & 1) ? __builtin_amdgcn_perm(0u, p[e >> 1], 0x020C000Cu) : p[e >> 1];
 }
 // store edge e's new pair r (int16 halves v << 8), keeping its partner's bytes
 template <int N>
@@ -163,7 +158,7 @@ k_pair_i8(const PairArgs a) {
           const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
           for (int c = 0; c < a.mcol; ++c) {
             const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
-            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24) | 0x00ff00ffu;
+            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
             wbase[c * a.col_dwords + pos] = p;
             wbase[c * a.col_dwords + pos + 64] = (p >> 16) | (p << 16);
           }
@@ -202,7 +197,7 @@ k_pair_i8(const PairArgs a) {
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
           const uint32_t x = *lds32(adr[e0 + j]);
-          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(c2v_sub_operand(c2vp, e0 + j)));
+          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(c2v_get(c2vp, e0 + j)));
           const short2v qq = as_s(as_u(qs) & 0xff00ff00u);   // Q << 8, Q = sat8(L - R)
           q[j] = qq;
           const ushort2v av = as_us(as_u(smax(qq, -qq)));
@@ -229,8 +224,8 @@ k_pair_i8(const PairArgs a) {
           const short2v m = smax(mv1, sumb - as_s(as_u(A[j])));   // argmin ? min2 : min1, capped
           const uint32_t sm = as_u(as_s(S ^ as_u(q[j])) >> short2v{15, 15});
           const uint32_t rn = as_u(as_us(as_u(m) ^ sm) - as_us(sm));
-          // L = sat8(Q + R), stored as (L << 8) | 0xff
-          const uint32_t g = as_u(__builtin_elementwise_add_sat(q[j], as_s(rn))) | 0x00ff00ffu;
+          // L = sat8(Q + R) in the high bytes
+          const uint32_t g = as_u(__builtin_elementwise_add_sat(q[j], as_s(rn)));
           const int e = e0 + j;   // compile-time after unrolling
           if ((e & 1) == 0 && j + 1 < w) pend = rn;
           else if ((e & 1) == 1 && j > 0) c2vp[e >> 1] = c2v_pack(pend, rn);   // both bytes new
