@@ -189,9 +189,9 @@ def main():
                          "one decode of --ncw codewords at -0.5 dB twice, then a plain copy of the LLR "
                          "tensor twice (counter calibration); 'sim' = --steps simulation loops; 'host' = "
                          "decode of --ncw codewords held in host memory (PCIe copies included), best of --steps")
-    ap.add_argument("--config", type=int, choices=[1, 2, 3, 4, 5], default=1,
+    ap.add_argument("--config", type=int, choices=[1, 2, 3, 4, 5, 6], default=1,
                     help="1: 11ad 1/2 int8 decode (the headline metric); 2: 16e 1/2 float sum-product Z sweep; "
-                         "3: mixed-code queue; 4: trainable decoder step; 5: 5G NR BG1 Z=384 int8")
+                         "3: mixed-code queue; 4: trainable decoder step; 5: 5G NR BG1 Z=384 int8; 6: 802.3ca int8")
     ap.add_argument("--z", default=None, help="config 2: comma-separated lifting sizes instead of 24..96")
     args = ap.parse_args()
     if args.probe:

@@ -256,5 +256,65 @@ def config5(args):
             "dtype": "int8", "data": "synthetic"}
 
 
+def config6(args):
+    """802.3ca (Z = 256, 12 x 69 blocks, K = 14592 of N = 17664), int8 offset min-sum beta 1
+    cap 31, max 10 iterations: simulation-loop CWER curves of the mother code and of the
+    code with its last two parity block columns punctured, and decode-only throughput."""
+    import torch
+    from . import codes
+    from . import lib as _L
+    from .decoder import Decoder, Hdec
+    from .sim import Henc, Sim
+    from .train import snr_at
+    ncw = 32768
+    loops = max(1, args.steps)
+    k = 14592
+    out = {"config": 6, "workload": "802.3ca LDPC (N=17664, K=14592, Z=256) int8 offset min-sum beta 1 cap 31, "
+                                    "max 10 iterations, 32768 cw per batch; mother code and last 512 parity bits "
+                                    "punctured", "dtype": "int8", "data": "synthetic"}
+    for punct in (False, True):
+        cs = codes.gen_structs("802.3ca", simd_size=512, elem_size=8, puncture_8023ca=punct)
+        henc = Henc.from_tables(cs.enc)
+        dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+        # Es/N0 dB around the waterfall (about 0.1 dB wide at 65536 cw per point)
+        snrs = [round((2.15 if not punct else 2.9) + 0.05 * i, 2) for i in range(8)]
+        sim = Sim(henc, dec, ncw, snrs, seed=100, use_graph=True)
+        sim.run(1)
+        torch.cuda.synchronize()
+        sim.reset()
+        t0 = time.perf_counter()
+        sim.run(loops)
+        st = sim.stats()
+        dt = time.perf_counter() - t0
+        curve = [(e, x["cw_err"] / x["cw"], x["bit_err"] / (x["cw"] * k), x["iters"] / x["cw"])
+                 for e, x in zip(snrs, st)]
+        tag = "punctured" if punct else "mother"
+        out[tag] = {"curve_esn0_cwer_ber_iters": curve, "esn0_at_cwer_1e-3": snr_at(curve),
+                    "sim_cw_per_s": ncw * len(snrs) * loops / dt}
+        if punct:
+            continue
+        pick = min(range(len(snrs)), key=lambda i: abs(math.log10(max(curve[i][1], 1e-9)) + 3))
+        bits = henc.new_bits(ncw)
+        henc.gen_codewords(bits, seed=7)
+        llr = torch.empty((ncw, henc.n), dtype=torch.int8, device="cuda")
+        henc.gen_llr(bits, llr, _L.I8, snrs[pick], seed=7)
+        post = torch.empty_like(llr)
+        it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        dec.decode(llr, post, iters=it)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 3
+        e0.record()
+        for _ in range(reps):
+            dec.decode(llr, post, iters=it)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out.update({"decode_at_esn0": snrs[pick], "decode_ms_per_batch": ms, "decode_cw_per_s": ncw / (ms * 1e-3),
+                    "decode_info_gbit_s": ncw * k / (ms * 1e-3) / 1e9,
+                    "decode_mean_iterations": float(it.float().mean())})
+    return out
+
+
 def run_config(n, args):
-    return {2: config2, 3: config3, 4: config4, 5: config5}[n](args)
+    return {2: config2, 3: config3, 4: config4, 5: config5, 6: config6}[n](args)
