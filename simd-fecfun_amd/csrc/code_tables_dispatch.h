@@ -3,6 +3,7 @@
 hipError_t code_dispatch(int code, const CodeArgs& a, hipStream_t st) {
   switch (code) {
   case 0: FECSIM_CODE_LAUNCH(CodeTab0)
+  case 1: FECSIM_CODE_LAUNCH(CodeTab1)
   default: return hipErrorInvalidValue;
   }
 }

@@ -14,7 +14,7 @@ from . import codes
 from .gen_pair_shapes import _write
 
 # large-Z codes: one codeword per workgroup of Z threads
-CODE_SETS = ["5GNR_n25344_k8448"]
+CODE_SETS = ["5GNR_n25344_k8448", "802.3ca"]
 LDS_LIMIT = 160 * 1024
 
 
