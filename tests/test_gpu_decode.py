@@ -316,6 +316,43 @@ def test_gpu_code_kernel_matches_oracle(case, cuda):
     _say("  iterations", ref[2])
 
 
+@pytest.mark.parametrize("case", [(2.3, 6, 10, 1.0, 31.0, False), (2.1, 4, 10, 2.0, 127.0, False),
+                                  (2.4, 4, 10, 127.0, 0.5, False), (2.3, 6, 6, 1.0, 31.0, True)],
+                         ids=lambda c: "esn0%g-n%d-it%d-b%g-c%g-tab%d" % c)
+def test_gpu_code_kernel_8023ca_matches_oracle(case, cuda):
+    """802.3ca (Z = 256, rows of weight 22-23) on the code-specialized int8 kernel against
+    the oracle: posteriors, C2V and iteration counts bit for bit, with plain offsets, wide
+    and zero caps, a per-(iteration, layer) offset table, and full-range / saturated
+    codewords."""
+    import torch
+    from fecsim import lib as L
+    esn0, ncw, mi, beta, cap, with_tab = case
+    _say("code kernel 802.3ca", case)
+    cs = codes.gen_structs("802.3ca", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, esn0, elem=O.I8, llr_scale=2.0, max_abs=24.0)
+    rng = np.random.default_rng(13)
+    llr[0] = rng.integers(-128, 128, cs.n)
+    llr[1] = np.where(rng.random(cs.n) < 0.5, -128, 127)
+    tab = (rng.integers(1, 4, (mi, cs.dec.mrow)).astype(np.float32) if with_tab else None)
+    ref = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=beta, beta_table=tab)
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, cap, beta)
+    if with_tab:
+        d.set_offsets(tab)
+    assert d.kernel == L.KERNEL_CODE
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    c2v = torch.zeros(ncw * cs.dec.total_hc_ones * cs.z, dtype=torch.int8, device="cuda")
+    d.decode(x, out, iters=iters, c2v=c2v)
+    torch.cuda.synchronize()
+    assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2])
+    assert np.array_equal(out.cpu().numpy(), ref[0])
+    assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1])
+    _say("  iterations", ref[2])
+
+
 def test_gpu_code_kernel_offsets_and_fallbacks(cuda):
     """Per-(iteration, layer) offset tables run on the code kernel when every entry is
     1..127 and match the oracle; a table with a 0 entry, beta 0, 16-bit elements and
