@@ -212,3 +212,18 @@ def test_exec_name_parse():
     assert codes.parse_exec_name("LDPCtop_Vec16f_prod") == (16, "f", "prod", 512, 32)
     with pytest.raises(ValueError):
         codes.parse_exec_name("LDPCtop_Vec64x_min")
+
+
+def test_code_tables_compiled_in():
+    """The codes compiled into the code-specialized int8 kernel are exactly the decoder
+    tables gen_structs writes (row relabelling included): BG1 Z=384 and 802.3ca, with
+    the LDS budget of three column copies checked."""
+    from fecsim import gen_code_tables
+    tb = {ms: rest for ms, *rest in gen_code_tables.tables()}
+    assert set(tb) == {"5GNR_n25344_k8448", "802.3ca"}
+    for ms, (z, mcol, w, col, sh) in tb.items():
+        d = codes.gen_structs(ms, simd_size=512, elem_size=8).dec
+        assert (z, mcol) == (d.z, d.mcol)
+        assert w == list(d.row_weights) and col == list(d.row_active_cols) and sh == list(d.row_offset)
+        assert (3 * mcol + 1) * z <= gen_code_tables.LDS_LIMIT
+    assert tb["802.3ca"][:2] == [256, 69] and len(tb["802.3ca"][3]) == 275
