@@ -205,6 +205,50 @@ def test_prngs_match_published_algorithms(oracle):
     assert _xoshiro([1, 2, 3, 4]) == (5 << 23) + 1
 
 
+# xoshiro256++ from the state {1, 2, 3, 4}: the published test vector of the reference
+# implementation (prng.di.unimi.it/xoshiro256plusplus.c), as the rand_xoshiro crate's
+# test_vector checks it
+XOSHIRO256PP_1234 = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+                     9973669472204895162, 14011001112246962877, 12406186145184390807, 15849039046786891736,
+                     10450023813501588000]
+
+
+def test_oracle_xoshiro_known_answer(oracle):
+    """orc_xoshiro_N (prng_functions.cpp:44-61) seeded with {1, 2, 3, 4} in every lane
+    (rand_state[16 + 4 i + lane] = s_i, init_randstate :22-26) gives the published vector."""
+    rs = np.zeros(64, np.uint64)
+    for i in range(4):
+        rs[16 + 4 * i: 16 + 4 * i + 4] = i + 1
+    oracle.lib().orc_init_randstate(rs.ctypes.data)
+    out = np.zeros(4, np.uint64)
+    got = []
+    for _ in range(len(XOSHIRO256PP_1234)):
+        oracle.lib().orc_xoshiro_N(1, out.ctypes.data)
+        assert len(set(int(x) for x in out)) == 1
+        got.append(int(out[0]))
+    assert got == XOSHIRO256PP_1234
+
+
+def test_oracle_sfc64_equals_numpy_sfc64(oracle):
+    """orc_sfc64_N (prng_functions.cpp:33-41) against numpy's own SFC64 bit generator
+    (PractRand's sfc64, an independent implementation) from the same per-lane states
+    (a, b, c, counter) = rand_state[4 i + lane], i = 0..3."""
+    seeds = refsim.perl_seeds(100)
+    oracle.lib().orc_init_randstate(seeds.ctypes.data)
+    gens = []
+    for lane in range(4):
+        bg = np.random.SFC64()
+        st = bg.state
+        st["state"]["state"] = np.array([seeds[4 * i + lane] for i in range(4)], dtype=np.uint64)
+        st["has_uint32"], st["uinteger"] = 0, 0
+        bg.state = st
+        gens.append(bg.random_raw(300))
+    out = np.zeros(4, np.uint64)
+    for k in range(300):
+        oracle.lib().orc_sfc64_N(0, out.ctypes.data)
+        assert [int(x) for x in out] == [int(g[k]) for g in gens], k
+
+
 def test_box_muller_pieces_accuracy(oracle):
     L = oracle.lib()
     xs = np.concatenate([np.geomspace(1e-19, 1.0, 2000, endpoint=False), np.linspace(0.01, 0.999, 500)])
