@@ -163,6 +163,19 @@ int fecsim_multi_autotune(fecsim_multi *m, uint32_t num_cw, const int32_t *code,
                           const void *llr_in, void *post_out, int rounds, void *stream);
 int fecsim_multi_shape(const fecsim_multi *m, int code, int *rows_per_lane, int *cw_per_wave,
                        int *waves_per_block);
+/* fecsim_multi_decode without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc): the queue
+ * workspace (grouping permutation, per-code counters, work counter) is a static device
+ * array of FECSIM_MULTI_STATIC_MAX_CW entries instead of the handle's, so num_cw may
+ * exceed the handle's max_cw up to that bound; larger queues return FECSIM_E_INVAL.
+ * Results equal fecsim_multi_decode's.  Every call on one GPU shares that GPU's array
+ * (each GPU holds its own copy, 4 MiB, until the process ends): calls that overlap on
+ * one GPU -- concurrent streams or host threads -- race and must be serialised by the
+ * caller.  No counterpart in the reference. */
+#define FECSIM_MULTI_STATIC_MAX_CW 1048576u
+int fecsim_multi_decode_static(fecsim_multi *m, uint32_t num_cw, const int32_t *code, const unsigned long long *off,
+                               const void *llr_in, void *post_out, uint32_t *iters, unsigned long long *tot_iter,
+                               void *stream);
 void fecsim_multi_free(fecsim_multi *m);
 
 

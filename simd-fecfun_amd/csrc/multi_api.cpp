@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -13,6 +14,37 @@
 using namespace fecsim;
 
 namespace {
+
+// Workspace of fecsim_multi_decode_static: the queue permutation (FECSIM_MULTI_STATIC_MAX_CW
+// entries) and the per-code counters (histogram, scatter cursors, codeword and work-item
+// starts for up to 64 codes, the work counter), 4 MiB + 1036 bytes.  Every GPU the
+// library runs on holds its own copy until the process ends; all static calls on one GPU
+// share it.
+constexpr uint32_t kStaticMaxCw = FECSIM_MULTI_STATIC_MAX_CW;
+constexpr size_t kStaticWords = (size_t)kStaticMaxCw + 64 + 64 + 65 + 65 + 1;
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
+// Device address of `scratch` per device, looked up once with that device current (in host
+// code the symbol's name is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+int static_workspace(uint32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(FECSIM_E_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FECSIM_OK;
+}
 
 void free_multi(fecsim_multi* m) {
   (void)hipFree(m->d_cores);
@@ -45,8 +77,9 @@ int update(fecsim_multi* m) {
   return FECSIM_OK;
 }
 
-MultiArgs args(const fecsim_multi* m, uint32_t num_cw, const int32_t* code, const unsigned long long* off,
-               const void* in, void* out, uint32_t* iters, unsigned long long* tot) {
+// ws: [ws_cw] queue permutation followed by the per-code counters
+MultiArgs args(const fecsim_multi* m, uint32_t* ws, uint32_t ws_cw, uint32_t num_cw, const int32_t* code,
+               const unsigned long long* off, const void* in, void* out, uint32_t* iters, unsigned long long* tot) {
   const int n = (int)m->decs.size();
   MultiArgs a{};
   a.cores = m->d_cores;
@@ -58,8 +91,8 @@ MultiArgs args(const fecsim_multi* m, uint32_t num_cw, const int32_t* code, cons
   a.out = out;
   a.iters = iters;
   a.tot_iter = tot;
-  a.perm = m->d_ws;
-  a.cnt = a.perm + m->max_cw;
+  a.perm = ws;
+  a.cnt = a.perm + ws_cw;
   a.cursor = a.cnt + n;
   a.code_start = a.cursor + n;
   a.item_start = a.code_start + n + 1;
@@ -72,9 +105,10 @@ MultiArgs args(const fecsim_multi* m, uint32_t num_cw, const int32_t* code, cons
   return a;
 }
 
-int run(fecsim_multi* m, uint32_t num_cw, const int32_t* code, const unsigned long long* off, const void* in,
-        void* out, uint32_t* iters, unsigned long long* tot, hipStream_t st) {
-  const MultiArgs a = args(m, num_cw, code, off, in, out, iters, tot);
+int run(fecsim_multi* m, uint32_t* ws, uint32_t ws_cw, uint32_t num_cw, const int32_t* code,
+        const unsigned long long* off, const void* in, void* out, uint32_t* iters, unsigned long long* tot,
+        hipStream_t st) {
+  const MultiArgs a = args(m, ws, ws_cw, num_cw, code, off, in, out, iters, tot);
   // persistent grid, no larger than the work could use
   uint32_t min_cpw = 64;
   for (const DecShape& s : m->shapes) min_cpw = std::min<uint32_t>(min_cpw, (uint32_t)s.cw_per_wave);
@@ -96,7 +130,7 @@ double time_queue(fecsim_multi* m, uint32_t num_cw, const int32_t* code, const u
   double best = std::numeric_limits<double>::infinity();
   for (int r = 0; r < rounds + 1; ++r) {   // the first run warms up
     (void)hipEventRecord(e0, st);
-    if (run(m, num_cw, code, off, in, out, nullptr, nullptr, st)) {
+    if (run(m, m->d_ws, m->max_cw, num_cw, code, off, in, out, nullptr, nullptr, st)) {
       best = -1.0;
       break;
     }
@@ -187,7 +221,25 @@ int fecsim_multi_decode(fecsim_multi* m, uint32_t num_cw, const int32_t* code, c
   if (!m || ((!code || !off || !llr_in || !post_out) && num_cw)) return fail(FECSIM_E_INVAL, "null argument");
   if (num_cw > m->max_cw) return fail(FECSIM_E_INVAL, "queue longer than max_cw");
   if (!num_cw) return FECSIM_OK;
-  return run(m, num_cw, code, off, llr_in, post_out, iters, tot_iter, (hipStream_t)stream);
+  return run(m, m->d_ws, m->max_cw, num_cw, code, off, llr_in, post_out, iters, tot_iter, (hipStream_t)stream);
+}
+
+int fecsim_multi_decode_static(fecsim_multi* m, uint32_t num_cw, const int32_t* code, const unsigned long long* off,
+                               const void* llr_in, void* post_out, uint32_t* iters, unsigned long long* tot_iter,
+                               void* stream) {
+  if (!m || ((!code || !off || !llr_in || !post_out) && num_cw)) return fail(FECSIM_E_INVAL, "null argument");
+  if (num_cw > kStaticMaxCw)
+    return fail(FECSIM_E_INVAL, "queue longer than the static workspace (FECSIM_MULTI_STATIC_MAX_CW entries)");
+  if (!num_cw) return FECSIM_OK;
+  uint32_t* ws = nullptr;
+  int rc = static_workspace(&ws);
+  if (rc) return rc;
+  const hipStream_t st = (hipStream_t)stream;
+  // the histogram must start at zero: a fill kernel (no memset, so a captured graph
+  // replays it correctly); the cursors and the work counter are reset by k_multi_scan
+  const hipError_t e = launch_fill_u64(reinterpret_cast<unsigned long long*>(ws + kStaticMaxCw), 32, 0ull, st);
+  if (e != hipSuccess) return hip_fail(e, "static workspace clear");
+  return run(m, ws, kStaticMaxCw, num_cw, code, off, llr_in, post_out, iters, tot_iter, st);
 }
 
 int fecsim_multi_autotune(fecsim_multi* m, uint32_t num_cw, const int32_t* code, const unsigned long long* off,

@@ -29,6 +29,14 @@ class MultiDecoder:
         _L.check(_L.lib().fecsim_multi_decode(self._m, n, _ptr(code), _ptr(off), _ptr(llr_in), _ptr(post_out),
                                               _ptr(iters), _ptr(tot_iter), _stream(stream)), "fecsim_multi_decode")
 
+    def decode_static(self, code, off, llr_in, post_out, iters=None, tot_iter=None, stream=None):
+        """decode() through fecsim_multi_decode_static: the queue workspace is the library's
+        static device array (no allocation; one call at a time per GPU)."""
+        n = code.numel()
+        _L.check(_L.lib().fecsim_multi_decode_static(self._m, n, _ptr(code), _ptr(off), _ptr(llr_in),
+                                                     _ptr(post_out), _ptr(iters), _ptr(tot_iter), _stream(stream)),
+                 "fecsim_multi_decode_static")
+
     def autotune(self, code, off, llr_in, post_out, rounds: int = 3, stream=None):
         _L.check(_L.lib().fecsim_multi_autotune(self._m, code.numel(), _ptr(code), _ptr(off), _ptr(llr_in),
                                                 _ptr(post_out), rounds, _stream(stream)), "fecsim_multi_autotune")
