@@ -92,6 +92,8 @@ int fecsim_decoder_create(const fecsim_hdec *h, const char *kind, const fecsim_d
  * identical results.  No reference counterpart (the reference selects its decoder at
  * compile time only). */
 enum { FECSIM_KERNEL_AUTO = 0, FECSIM_KERNEL_GENERIC = 1, FECSIM_KERNEL_PAIR = 2, FECSIM_KERNEL_CODE = 3 };
+/* FECSIM_E_INVAL while a queue decoder (fecsim_multi_create) or a simulation loop
+ * (fecsim_sim_create) holds d, as fecsim_decoder_set_offsets */
 int fecsim_decoder_select_kernel(fecsim_decoder *d, int kernel);
 /* kernel a dense decode_set call on aligned buffers would run now
  * (FECSIM_KERNEL_GENERIC, _PAIR or _CODE) */
@@ -106,7 +108,10 @@ int fecsim_decoder_info(const fecsim_decoder *d, int *elem, int *mode, int *elem
  * beta_offset.  NULL restores the constant offset.  No reference counterpart: this is
  * how offsets learned with the trainable decoder below are run in the bit-exact
  * decoders; with every entry equal to beta_offset the results are the plain
- * decoder's.  A decoder with a table runs the generic kernel. */
+ * decoder's.  A decoder with a table runs the generic kernel.  Returns FECSIM_E_INVAL
+ * while a queue decoder (fecsim_multi_create) or a simulation loop (fecsim_sim_create)
+ * holds d: they copy its tables and kernel choice when they are made, so set offsets
+ * before creating them (or after freeing them). */
 int fecsim_decoder_set_offsets(fecsim_decoder *d, const float *beta);
 void fecsim_decoder_free(fecsim_decoder *d);
 

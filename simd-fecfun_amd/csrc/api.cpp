@@ -245,6 +245,8 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
 
 int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
+  if (d->holders.load() > 0)
+    return fail(FECSIM_E_INVAL, "decoder is held by a queue decoder or simulation loop; select its kernel first");
   if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_CODE) return fail(FECSIM_E_INVAL, "unknown kernel");
   if (kernel == FECSIM_KERNEL_PAIR && d->d.pair_shape < 0)
     return fail(FECSIM_E_UNSUPPORTED, "no row-pair kernel for this code / element / parameters");
@@ -270,6 +272,8 @@ int fecsim_decoder_kernel(const fecsim_decoder* d) {
 
 int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
+  if (d->holders.load() > 0)
+    return fail(FECSIM_E_INVAL, "decoder is held by a queue decoder or simulation loop; set its offsets first");
   DecoderDev& dv = d->d;
   if (!beta) {
     if (dv.d_beta_tab) (void)hipFree(dv.d_beta_tab);

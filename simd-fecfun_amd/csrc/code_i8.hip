@@ -23,6 +23,8 @@
 //   written since the last barrier are issued before it.  The syndrome after each
 //   iteration is the XOR of each row's biased bytes, and a workgroup OR ends the loop
 //   (src/LDPCDec/LDPCdecode.cpp:172-197).
+#include <mutex>
+
 #include "decode_core.h"
 #include "code_tables.h"
 
@@ -325,14 +327,22 @@ template <class CT>
 hipError_t launch_code(const CodeArgs& a, hipStream_t st) {
   if (a.num_cw == 0) return hipSuccess;
   constexpr size_t lds = (size_t)(3 * CT::mcol + 1) * CT::z;
-  static bool ready = false;
-  if (!ready) {
-    if (lds > 65536) {
-      const hipError_t e = hipFuncSetAttribute((const void*)k_code_i8<CT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)lds);
-      if (e != hipSuccess) return e;
+  // the LDS attribute is set once per device (mutex: host threads may launch
+  // concurrently on distinct handles)
+  static std::mutex mu;
+  static bool ready_by_dev[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ready_by_dev[dev]) {
+      if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_code_i8<CT>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+      }
+      ready_by_dev[dev] = true;
     }
-    ready = true;
   }
   // one workgroup per codeword: the dispatcher hands a finished workgroup's CU the next
   // codeword, so codewords that stop early free their CU at once

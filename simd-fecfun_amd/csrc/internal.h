@@ -1,6 +1,8 @@
 // Internal structures shared by the C ABI (api.cpp) and the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <stdint.h>
 
 #include <string>
@@ -272,6 +274,10 @@ inline void layout_strides(int z, int mcol, int layout_p, long long* cw_stride, 
 
 struct fecsim_decoder {
   fecsim::DecoderDev d;
+  // queue decoders and simulation loops holding this decoder: they keep its tables,
+  // offset table and kernel choice (copied into device cores or a captured graph), so
+  // those cannot change while holders > 0
+  std::atomic<int> holders{0};
 };
 
 struct fecsim_hdec {

@@ -211,6 +211,7 @@ int fecsim_multi_create(fecsim_decoder* const* decs, int ncode, uint32_t max_cw,
     delete m;
     return rc;
   }
+  for (fecsim_decoder* d : m->decs) d->holders.fetch_add(1);
   *out = m;
   return FECSIM_OK;
 }
@@ -299,6 +300,7 @@ int fecsim_multi_shape(const fecsim_multi* m, int c, int* rows_per_lane, int* cw
 
 void fecsim_multi_free(fecsim_multi* m) {
   if (!m) return;
+  for (fecsim_decoder* d : m->decs) d->holders.fetch_sub(1);
   free_multi(m);
   delete m;
 }

@@ -28,6 +28,8 @@
 //   sat8(Q + R) and whose low byte is 0 or, at 32767, 0xff.
 // * |Q| is max(Q, -Q) read as unsigned 16-bit, so |-128 << 8| = 0x8000 = 128 << 8
 //   exactly; R = (S xor sign Q) ? -M : M.
+#include <mutex>
+
 #include "decode_core.h"
 #include "pair_shapes.h"
 
@@ -331,19 +333,28 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
   // codewords with per-slot refill: chunks of ~30 codewords per wave keep the spread of
   // iteration counts from leaving a long tail (measured on 11ad 1/2: 2x -> 8x is +5.6%,
   // 16x loses again)
-  static int resident = -1;
-  if (resident < 0) {
-    // the twin addressing (byte address xor 256) needs the waves' columns at multiples
-    // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
-    hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH>) != hipSuccess || fa.sharedSizeBytes != 0)
-      return hipErrorInvalidDeviceFunction;
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH>, 64 * a.waves, lds) != hipSuccess)
-      return hipErrorInvalidValue;
-    resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+  // resident blocks per device, measured once per device (mutex: host threads may
+  // launch concurrently on distinct handles)
+  static std::mutex mu;
+  static int resident_by_dev[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  int resident;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!resident_by_dev[dev]) {
+      // the twin addressing (byte address xor 256) needs the waves' columns at multiples
+      // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
+      hipFuncAttributes fa;
+      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH>) != hipSuccess || fa.sharedSizeBytes != 0)
+        return hipErrorInvalidDeviceFunction;
+      int cus = 0, per_cu = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH>, 64 * a.waves, lds) != hipSuccess)
+        return hipErrorInvalidValue;
+      resident_by_dev[dev] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    }
+    resident = resident_by_dev[dev];
   }
   const long long slots_per_block = (long long)a.waves * a.cw_per_wave;
   long long blocks = (a.num_cw + slots_per_block - 1) / slots_per_block;

@@ -16,6 +16,7 @@ using namespace fecsim;
 struct fecsim_sim {
   fecsim_henc* h = nullptr;
   fecsim_decoder* d = nullptr;
+  bool holds_d = false;                     // counted in d->holders
   uint32_t num_cw = 0;
   std::vector<float> snr, sigma, factor;
   float llr_scale = 0, max_abs = 0;
@@ -412,6 +413,7 @@ int fecsim_count_errors(fecsim_henc* h, uint32_t num_cw, int col_words, const ui
 
 void fecsim_sim_free(fecsim_sim* s) {
   if (!s) return;
+  if (s->holds_d) s->d->holders.fetch_sub(1);
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
@@ -466,6 +468,8 @@ int fecsim_sim_create(fecsim_henc* h, fecsim_decoder* d, const fecsim_sim_params
     fecsim_sim_free(s);
     return hip_fail(e, "simulation buffers");
   }
+  d->holders.fetch_add(1);
+  s->holds_d = true;
   *out = s;
   return FECSIM_OK;
 }

@@ -208,3 +208,36 @@ def test_gpu_mixed_sim_loop_counts(graph, cuda):
     assert sum(s["cw_err"] for s in st) == tot0_cw + tot_cw_err
     assert sum(s["bit_err"] for s in st) == tot0_bit + tot_bit_err
     _say("mixed sim", graph, "cw errors", [s["cw_err"] for s in st], "iterations", iters)
+
+
+def test_gpu_held_decoder_refuses_offset_and_kernel_changes(cuda):
+    """A queue decoder and a simulation loop copy their decoders' tables and kernel
+    choice when made; while they exist the decoder refuses set_offsets / select_kernel
+    (FECSIM_E_INVAL) and their results stay those of the decoder as it was."""
+    import torch
+    from fecsim import lib as L
+    from fecsim.sim import Sim
+    n = 600
+    code, offs, packed, parts, css, decs = _queue(n, seed=77)
+    outs, its = _expected(parts, decs)
+    m = MultiDecoder(decs, max_cw=n)
+    tab = np.full((decs[0].max_iter, css[0].dec.mrow), 2.0, np.float32)
+    for bad in (lambda: decs[0].set_offsets(tab), lambda: decs[0].set_offsets(None),
+                lambda: decs[1].select_kernel(L.KERNEL_GENERIC)):
+        with pytest.raises(L.FecsimError) as ei:
+            bad()
+        assert ei.value.status == -1
+    post = torch.zeros_like(packed)
+    iters = torch.zeros(n, dtype=torch.int32, device="cuda")
+    m.decode(torch.from_numpy(code).cuda(), torch.from_numpy(offs).cuda(), packed, post, iters=iters)
+    torch.cuda.synchronize()
+    _check(code, offs, post, iters, outs, its)
+    del m
+    decs[0].set_offsets(tab)          # free again
+    decs[0].set_offsets(None)
+    cs = css[0]
+    sim = Sim(Henc.from_tables(cs.enc), decs[0], 64, [1.0], seed=3)
+    with pytest.raises(L.FecsimError):
+        decs[0].set_offsets(tab)
+    del sim
+    decs[0].set_offsets(tab)
