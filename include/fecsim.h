@@ -143,6 +143,12 @@ int fecsim_decode(fecsim_decoder *d, uint32_t num_cw, int layout_p, const void *
 int fecsim_decode_set_host(fecsim_decoder *d, uint32_t num_cw, int layout_p, void *metrics_host,
                            uint32_t *iters_host, unsigned long long *tot_iter);
 
+/* The float32 sum-product decoder's own device functions, elementwise over device
+ * arrays, for verifying their accuracy: fn 0 gives tanh(x[i] / 2) (x >= 0), fn 1 gives
+ * 2 atanh(x[i]) (0 <= x < 1).  They stand where the reference's generated product
+ * functions call vectorclass tanh / atanh (src/LDPCDec/min_find.pl:39,45). */
+int fecsim_sp_math(int fn, const float *x, float *y, uint32_t n, void *stream);
+
 /* ------------------------------------------------------------------------- */
 /* Mixed-code queue decoder (no reference counterpart)                        */
 /* ------------------------------------------------------------------------- */

@@ -384,4 +384,11 @@ int fecsim_decode_set_host(fecsim_decoder* dec, uint32_t num_cw, int layout_p, v
   return FECSIM_OK;
 }
 
+int fecsim_sp_math(int fn, const float* x, float* y, uint32_t n, void* stream) {
+  if (fn != 0 && fn != 1) return fail(FECSIM_E_INVAL, "fn must be 0 (tanh(x/2)) or 1 (2 atanh(x))");
+  if ((!x || !y) && n) return fail(FECSIM_E_INVAL, "null argument");
+  const hipError_t he = launch_sp_math(fn, x, y, n, (hipStream_t)stream);
+  return he == hipSuccess ? FECSIM_OK : hip_fail(he, "sp_math launch");
+}
+
 }  // extern "C"

@@ -471,4 +471,17 @@ hipError_t launch_decode(const DecoderDev& d, const KernelArgs& a, hipStream_t s
   }
 }
 
+namespace {
+__global__ void k_sp_math(int fn, const float* __restrict__ x, float* __restrict__ y, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = fn == 0 ? sp_tanh_half(x[i]) : sp_two_atanh(x[i]);
+}
+}  // namespace
+
+hipError_t launch_sp_math(int fn, const float* x, float* y, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sp_math, dim3((n + 255) / 256), dim3(256), 0, st, fn, x, y, n);
+  return hipGetLastError();
+}
+
 }  // namespace fecsim

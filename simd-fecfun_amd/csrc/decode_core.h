@@ -322,6 +322,14 @@ __device__ __forceinline__ bool row_parity(const typename P::S* __restrict__ pos
 // one check row, float elements (min-sum or sum-product)
 // ---------------------------------------------------------------------------
 
+// The sum-product factors tanh(A/2) and messages 2 atanh(p): float32 from the hardware
+// approximations (within 4 units of the correctly rounded values), double from the
+// series that the oracle restates bit for bit.
+__device__ __forceinline__ float sp_tanh_half(float a) { return fecmath::tanh_half_fast(a); }
+__device__ __forceinline__ double sp_tanh_half(double a) { return fecmath::tanh_own(a * 0.5); }
+__device__ __forceinline__ float sp_two_atanh(float p) { return fecmath::two_atanh_fast(p); }
+__device__ __forceinline__ double sp_two_atanh(double p) { return 2.0 * fecmath::atanh_own(p); }
+
 template <class P, int MAXW>
 __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, typename P::S* __restrict__ c2v,
                                           const uint32_t* __restrict__ edges, int e0, int w, int r, int Z,
@@ -345,13 +353,13 @@ __device__ __forceinline__ void row_float(typename P::S* __restrict__ post, type
   if constexpr (P::kProd) {
     F t[MAXW];
 #pragma unroll
-    for (int j = 0; j < MAXW; ++j) t[j] = (j < w) ? fecmath::tanh_own(A[j] * F(0.5)) : F(0);
+    for (int j = 0; j < MAXW; ++j) t[j] = (j < w) ? sp_tanh_half(A[j]) : F(0);
     F o[MAXW];
 #pragma unroll
     for (int j = 0; j < MAXW; ++j) o[j] = F(0);
     prod_tree_dyn<F>(w, t, cap, o);
 #pragma unroll
-    for (int j = 0; j < MAXW; ++j) M[j] = (j < w) ? F(2) * fecmath::atanh_own(o[j]) : F(0);
+    for (int j = 0; j < MAXW; ++j) M[j] = (j < w) ? sp_two_atanh(o[j]) : F(0);
   } else {
     F m1 = F(INFINITY), m2 = F(INFINITY);
     int im = -1;
@@ -420,10 +428,10 @@ __device__ __forceinline__ void row_float_w(typename P::S* __restrict__ post, ty
   if constexpr (P::kProd) {
     F t[W], o[W];
 #pragma unroll
-    for (int j = 0; j < W; ++j) t[j] = fecmath::tanh_own(A[j] * F(0.5));
+    for (int j = 0; j < W; ++j) t[j] = sp_tanh_half(A[j]);
     prod_tree_dyn<F>(W, t, cap, o);
 #pragma unroll
-    for (int j = 0; j < W; ++j) M[j] = F(2) * fecmath::atanh_own(o[j]);
+    for (int j = 0; j < W; ++j) M[j] = sp_two_atanh(o[j]);
   } else {
     F m1 = F(INFINITY), m2 = F(INFINITY);
     int im = -1;

@@ -1,11 +1,13 @@
 // Device math for the sum-product decoder and the noise generator.
 //
-// tanh/atanh: the reference evaluates them with vectorclass (src/LDPCDec/min_find.pl:39,45);
-// these are written from the mathematics -- exp by Cody-Waite reduction plus a Taylor
-// polynomial, log from the series 2*atanh((m-1)/(m+1)), atanh from its Taylor series
-// below 0.25 and 0.5*log((1+x)/(1-x)) above.  The expressions are evaluated in a
-// fixed order (the library is compiled with -ffp-contract=off) so results are
-// reproducible bit for bit on the CPU.
+// tanh/atanh: the reference evaluates them with vectorclass (src/LDPCDec/min_find.pl:39,45).
+// The double decoder uses the forms written from the mathematics below -- exp by
+// Cody-Waite reduction plus a Taylor polynomial, log from the series
+// 2*atanh((m-1)/(m+1)), atanh from its Taylor series below 0.25 and
+// 0.5*log((1+x)/(1-x)) above -- evaluated in a fixed order (the library is compiled
+// with -ffp-contract=off), so its results are reproducible bit for bit on the CPU.
+// The float decoder uses tanh_half_fast / two_atanh_fast, built on the hardware
+// exp2 / log2 / reciprocal approximations.
 //
 // fastm2log / sincos_norm: the reference's own approximations of -2*ln(x) and of
 // sin/cos(pi*x) used in its Box-Muller transform (src/RNGfunc/prng_functions.cpp:81-141).
@@ -80,6 +82,47 @@ __device__ __forceinline__ float atanh_own(float x) {
     r = 0.5f * log_own((1.0f + a) / (1.0f - a));
   }
   return x < 0.0f ? -r : r;
+}
+
+// ---- float32 sum-product factors from the hardware exp2 / log2 / reciprocal ----
+//
+// The float32 decoder evaluates tanh(A/2) and 2 atanh(p) for every edge; these forms
+// take 13-15 instructions each, branch-free (both halves computed, one selected), in
+// place of the series with IEEE divisions above.  Accuracy: within 4 units in the
+// last place of the correctly rounded values (the polynomial halves within 2; the
+// hardware v_exp_f32 / v_log_f32 / v_rcp_f32 are 1-unit approximations).
+
+// tanh(a / 2) for a >= 0.  Below a = 1.5: x + x^3 q(x^2), x = a/2, q a degree-4
+// minimax fit of (tanh(x)/x - 1)/x^2 on [0, 0.75].  Above: (1 - e) / (1 + e) with
+// e = exp(-a) = 2^(-a log2(e)), where the cancellation in 1 - e costs nothing.
+__device__ __forceinline__ float tanh_half_fast(float a) {
+  const float x = a * 0.5f, s = x * x;
+  float q = -0.004716447088867426f;
+  q = __builtin_fmaf(q, s, 0.01958320289850235f);
+  q = __builtin_fmaf(q, s, -0.053344596177339554f);
+  q = __builtin_fmaf(q, s, 0.13325409591197968f);
+  q = __builtin_fmaf(q, s, -0.33332958817481995f);
+  const float tp = __builtin_fmaf(x * s, q, x);
+  const float e = __builtin_amdgcn_exp2f(a * -1.44269504f);
+  const float te = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
+  return a < 1.5f ? tp : te;
+}
+
+// 2 atanh(p) for 0 <= p < 1.  Below p = 0.5: y + y p^2 q(p^2), y = 2p, q a degree-6
+// minimax fit of (atanh(p)/p - 1)/p^2 on [0, 0.5].  Above: ln((1 + p) / (1 - p)) =
+// ln 2 * log2((1 + p) * (1 / (1 - p))), with 1 - p exact for p >= 0.5.
+__device__ __forceinline__ float two_atanh_fast(float p) {
+  const float s = p * p, y = p + p;
+  float q = 0.177582249045372f;
+  q = __builtin_fmaf(q, s, 0.022989485412836075f);
+  q = __builtin_fmaf(q, s, 0.1033991202712059f);
+  q = __builtin_fmaf(q, s, 0.1095632016658783f);
+  q = __builtin_fmaf(q, s, 0.14295953512191772f);
+  q = __builtin_fmaf(q, s, 0.1999967098236084f);
+  q = __builtin_fmaf(q, s, 0.3333333730697632f);
+  const float mp = __builtin_fmaf(y * s, q, y);
+  const float ml = __builtin_amdgcn_logf((1.0f + p) * __builtin_amdgcn_rcpf(1.0f - p)) * 0.693147182f;
+  return p < 0.5f ? mp : ml;
 }
 
 // ---- double versions (Vec8d_prod) ----

@@ -244,6 +244,8 @@ hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);
 hipError_t launch_count(int elem, const CountArgs& a, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t stream);
+// the float32 sum-product decoder's tanh(x/2) (fn 0) or 2 atanh(x) (fn 1), elementwise
+hipError_t launch_sp_math(int fn, const float* x, float* y, uint32_t n, hipStream_t stream);
 hipError_t launch_hard(int elem, const CountArgs& a, uint32_t* out, hipStream_t stream);
 hipError_t launch_iter_hist(const uint32_t* iters, uint32_t n, uint32_t nbins, unsigned long long* hist,
                             hipStream_t stream);

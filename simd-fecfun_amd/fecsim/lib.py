@@ -60,6 +60,7 @@ SIGNATURES = [
     ("fecsim_multi_create", _int, [_vp, _int, _u32, C.POINTER(_vp)]),
     ("fecsim_multi_decode", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("fecsim_multi_decode_static", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("fecsim_sp_math", _int, [_int, _vp, _vp, _u32, _vp]),
     ("fecsim_multi_autotune", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _int, _vp]),
     ("fecsim_multi_shape", _int, [_vp, _int, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
     ("fecsim_multi_free", None, [_vp]),

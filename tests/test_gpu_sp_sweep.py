@@ -57,3 +57,37 @@ def test_gpu_double_sum_product_matches_libm_restatement(z, cuda):
     rep = sp_ref.sp_parity(post, iters, want, want_it, peak, rtol=1e-12)
     assert rep["ok"], rep
     assert rep["well_conditioned"] >= rep["codewords"] // 2, rep
+
+
+def _ulps32(a, b):
+    ia = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    ib = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(ia - ib)
+
+
+def test_gpu_sp_device_functions_within_4_ulps(cuda):
+    """The float32 decoder's tanh(a/2) and 2 atanh(p) (fecsim_sp_math) against libm
+    rounded to float32, over the arguments the decoder feeds them: a = |Q| up to far
+    beyond saturation, p from 0 to the cap 1 - 6e-8 (and every float32 within 3000
+    units of 1 below the cap)."""
+    import torch
+    from fecsim import lib as L
+    rng = np.random.default_rng(3)
+    a = np.concatenate([np.geomspace(1e-30, 1e-2, 500), np.linspace(0, 40, 400001), rng.uniform(0, 20, 100000),
+                        [1.5, np.nextafter(np.float32(1.5), np.float32(0)), 1e6, 1e9]]).astype(np.float32)
+    cap = np.float32(1.0 - float(np.float32(6e-8)))
+    p = np.concatenate([np.geomspace(1e-30, 1e-2, 500), np.linspace(0, 0.9999, 400001), rng.uniform(0, 1, 100000),
+                        1.0 - np.arange(1, 3000) * 2.0 ** -24, [0.5, np.nextafter(np.float32(0.5), np.float32(0))]])
+    p = p.astype(np.float32)
+    p = p[(p >= 0) & (p <= cap)]
+    for fn, x, want in ((0, a, np.tanh(a.astype(np.float64) * 0.5).astype(np.float32)),
+                        (1, p, (2.0 * np.arctanh(p.astype(np.float64))).astype(np.float32))):
+        xt = torch.from_numpy(x).cuda()
+        yt = torch.empty_like(xt)
+        L.check(L.lib().fecsim_sp_math(fn, xt.data_ptr(), yt.data_ptr(), x.size, None), "fecsim_sp_math")
+        torch.cuda.synchronize()
+        got = yt.cpu().numpy()
+        d = _ulps32(got, want)
+        print("fn", fn, "max ulp", int(d.max()), "at", float(x[d.argmax()]), file=sys.stderr, flush=True)
+        assert d.max() <= 4, (fn, x[d.argmax()], got[d.argmax()], want[d.argmax()])
+        assert np.all(got >= 0)
