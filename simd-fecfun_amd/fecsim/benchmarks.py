@@ -31,6 +31,37 @@ def _rate_db(cs):
     return 10 * math.log10((cs.enc.mcol - cs.enc.mrow) / cs.enc.mcol)
 
 
+# VALU roofs of one MI355X: 256 CUs x 64 lanes x 2.4 GHz 32-bit lane-ops per second; the
+# float and int32-lane kernels issue no packed (two-lane) instructions
+VALU32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# algorithmic work per edge-update (one edge of one check row in one iteration):
+# offset min-sum 13 int ops (residual with saturation, |.|, -beta, max 0, sign xor; exclusive
+# min 3; sign select, negate, cap, saturating add; syndrome xor); sum-product 15 float ops and
+# 2 transcendentals (tanh, atanh), counted as one op each
+OPS_OMS, OPS_SP = 13, 17
+
+
+def _timed(fn, stream, reps=1):
+    """average milliseconds of fn() launched reps times on stream (HIP events on that stream)"""
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _roofline(kernel, edge_updates, ms, ops, note, launches=1):
+    """VALU roofline object: algorithmic ops over the measured kernel time"""
+    tops = edge_updates * ops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": tops, "peak": VALU32_PEAK_TOPS, "unit": "Tops/s",
+            "frac": tops / VALU32_PEAK_TOPS, "ops_per_edge_update": ops,
+            "edge_updates_per_launch": edge_updates / launches, "avg_launch_ms": ms / launches,
+            "traffic": None, "note": note}
+
+
 def config2(args):
     import torch
     from . import codes
@@ -39,6 +70,7 @@ def config2(args):
     ebn0 = [0.5, 1.0, 1.5, 2.0, 2.5]
     ncw = 1 << 18
     per_z, tot_cw, tot_t = [], 0, 0.0
+    roof_eu, roof_ms = 0.0, 0.0
     zs = [int(x) for x in args.z.split(",")] if getattr(args, "z", None) else list(range(24, 97, 4))
     for z in zs:
         cs = codes.gen_structs("16e 1/2", simd_size=512, elem_size=32, z_over=z)
@@ -60,8 +92,26 @@ def config2(args):
                       "mean_iterations": [x["iters"] / x["cw"] for x in st]})
         tot_cw += ncw * len(snrs)
         tot_t += dt
-        _log("config 2: z=%d %.3f s cwer %s" % (z, dt, ["%.2e" % c for c in per_z[-1]["cwer"]]))
-    return {"config": 2, "workload": "16e 1/2 Z=24..96 step 4, Vec16f_prod (float32 sum-product), max 10 iterations, "
+        # decode-only launch at Eb/N0 1.5 dB (the middle point) for the roofline
+        from . import lib as _L
+        bits = henc.new_bits(ncw)
+        henc.gen_codewords(bits, seed=7)
+        llr = torch.empty((ncw, henc.n), dtype=torch.float32, device="cuda")
+        henc.gen_llr(bits, llr, _L.F32, 1.5 + _rate_db(cs), seed=7, llr_scale=1.0, max_abs_llr=1e9)
+        out = torch.empty_like(llr)
+        it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        st_ = torch.cuda.current_stream()
+        dec.decode(llr, out, iters=it, stream=st_)
+        ms = _timed(lambda: dec.decode(llr, out, iters=it, stream=st_), st_)
+        eu = float(it.double().sum().item()) * cs.dec.total_hc_ones * cs.z
+        roof_eu += eu
+        roof_ms += ms
+        del bits, llr, out
+        _log("config 2: z=%d %.3f s cwer %s; decode at 1.5 dB %.2f ms" % (z, dt, ["%.2e" % c for c in per_z[-1]["cwer"]],
+                                                                          ms))
+    return {"config": 2, "roofline": _roofline(
+                "k_decode<FloatPolicy<float,true>,8> (Z<=64) / k_decode_wg<...> (Z>=68)", roof_eu, roof_ms, OPS_SP,
+                "decode-only launches of 2^18 cw at Eb/N0 1.5 dB, one per Z, summed", launches=len(zs)), "workload": "16e 1/2 Z=24..96 step 4, Vec16f_prod (float32 sum-product), max 10 iterations, "
                                      "Eb/N0 0.5..2.5 dB step 0.5, 2^18 cw per point, simulation loop per Z",
             "ebn0_db": ebn0, "cw_per_s": tot_cw / tot_t, "codewords": tot_cw, "seconds": tot_t, "per_z": per_z,
             "dtype": "f32", "data": "synthetic"}
@@ -99,6 +149,21 @@ def config3(args):
     _L.check(_L.lib().fecsim_multi_autotune(m._m, n, code, off, llr, C.c_void_p(scratch.data_ptr()), 3,
                                             C.c_void_p(torch.cuda.current_stream().cuda_stream)), "autotune")
     shapes1 = [m.shape(c) for c in range(len(CODES3))]
+    # decode-only launch of the tuned queue for the roofline (k_multi_decode, sort kernels included)
+    qcode, qoff, _, _ = tune.queue()
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    st_ = torch.cuda.current_stream()
+    sp = C.c_void_p(st_.cuda_stream)
+
+    def _dec():
+        _L.check(_L.lib().fecsim_multi_decode(m._m, n, code, off, llr, C.c_void_p(scratch.data_ptr()),
+                                              C.c_void_p(it.data_ptr()), None, sp), "multi decode")
+    _dec()
+    ms = _timed(_dec, st_, reps=3)
+    e_code = torch.tensor([cs.dec.total_hc_ones * cs.z for cs in css], dtype=torch.float64, device="cuda")
+    eu = float((it.double() * e_code[qcode.long()]).sum().item())
+    roof = _roofline("k_multi_decode<IntPolicy<8>,24>", eu, ms, OPS_OMS,
+                     "decode-only launches of the autotuned 65536-entry queue (grouping sort included)")
     del tune
     sim = MixedSim(hencs, m, n, ebn0=(1.0, 4.0), seed=100, use_graph=True)
     sim.run(1)
@@ -110,7 +175,7 @@ def config3(args):
     st, iters = sim.stats()
     dt = time.perf_counter() - t0
     info_bits = sum(s["cw"] * (cs.enc.mcol - cs.enc.mrow) * cs.z for s, cs in zip(st, css))
-    return {"config": 3, "workload": "mixed queue of 65536 cw per loop over 5 codes (11n 1/2 z27, 11n 3/4 z54, "
+    return {"config": 3, "roofline": roof, "workload": "mixed queue of 65536 cw per loop over 5 codes (11n 1/2 z27, 11n 3/4 z54, "
                                      "11n 5/6 z81, 11ad 13/16, 16e 2/3A Z=96), Eb/N0 U[1,4] dB, int8 min-sum, max "
                                      "iterations 8/12/16/20/10, HIP graph per loop (encode+WGN+decode+count)",
             "loops": loops, "seconds": dt, "cw_per_s": n * loops / dt, "info_gbit_s": info_bits / dt / 1e9,
@@ -245,7 +310,10 @@ def config5(args):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    return {"config": 5, "workload": "5G NR BG1 Z=384 (K=8448, N=25344 sent, 2Z punctured), int8 offset min-sum "
+    roof = _roofline("k_code_i8<CodeTab0>" if dec.kernel == _L.KERNEL_CODE else "k_decode_wg<IntPolicy<8>,...>",
+                     float(it.double().sum().item()) * cs.dec.total_hc_ones * cs.z, ms, OPS_OMS,
+                     "decode-only launch of 16384 cw at Eb/N0 %.2f dB" % ebn0[pick])
+    return {"config": 5, "roofline": roof, "workload": "5G NR BG1 Z=384 (K=8448, N=25344 sent, 2Z punctured), int8 offset min-sum "
                                      "beta 1 cap 31, max 20 iterations, 16384 cw per batch, Eb/N0 0..1.5 dB",
             "sim_seconds": dt, "sim_cw_per_s": ncw * len(snrs) * loops / dt,
             "curve_ebn0_cwer_ber_iters": curve, "ebn0_at_cwer_1e-3": e3,
