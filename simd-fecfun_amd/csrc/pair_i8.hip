@@ -100,13 +100,18 @@ constexpr int pair_waves_per_eu() {
   return (SH::ne + 1) / 2 + SH::ne + 3 * SH::maxw + 20 <= 128 ? 4 : 1;
 }
 
-template <class SH>
+// ZC: the lifting size compiled in (SH::zc, with SH::mc columns), or 0 for a run-time Z.
+// With ZC the codeword's loads and stores take immediate offsets from one base address
+// and its column loops unroll (a refill or write-out costs 1-2 VALU per column instead of
+// the 64-bit address arithmetic of each byte).
+template <class SH, int ZC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<SH>())))
 k_pair_i8(const PairArgs a) {
   constexpr int NE = SH::ne;
+  constexpr bool FZ = ZC != 0;
   HIP_DYNAMIC_SHARED(uint32_t, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int L = a.half, Z = a.z, CPW = a.cw_per_wave;
+  const int L = FZ ? ZC / 2 : a.half, Z = FZ ? ZC : a.z, CPW = a.cw_per_wave;
   const bool real = lane < CPW * L;          // lanes past the last slot mirror lane 0
   const int slot = real ? lane / L : 0;
   const int lr = real ? lane - slot * L : 0;
@@ -157,12 +162,36 @@ k_pair_i8(const PairArgs a) {
         active = true;
         iter = 0;
         if (real) {
-          const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
-          for (int c = 0; c < a.mcol; ++c) {
-            const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
-            const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
-            wbase[c * a.col_dwords + pos] = p;
-            wbase[c * a.col_dwords + pos + 64] = (p >> 16) | (p << 16);
+          if constexpr (FZ) {
+            // columns in groups of four: eight byte loads at immediate offsets, then per
+            // column two byte permutes, the pair (lo, hi) and its twin (hi, lo) as int16
+            // halves v << 8 (groups bound the registers live beside the decode state)
+            const uint8_t* g = reinterpret_cast<const uint8_t*>(a.llr_in) + cw * (long long)(SH::mc * ZC) + lr;
+            uint32_t* wp = wbase + pos;
+            static_assert(SH::mc % 4 == 0, "column groups of four");
+#pragma unroll
+            for (int c0 = 0; c0 < SH::mc; c0 += 4) {
+              uint32_t lo[4], hi[4];
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                lo[k] = g[(c0 + k) * ZC];
+                hi[k] = g[(c0 + k) * ZC + ZC / 2];
+              }
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                wp[(c0 + k) * 128] = __builtin_amdgcn_perm(hi[k], lo[k], 0x040c000cu);
+                wp[(c0 + k) * 128 + 64] = __builtin_amdgcn_perm(lo[k], hi[k], 0x040c000cu);
+              }
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          } else {
+            const int8_t* g = a.llr_in + cw * (long long)(a.mcol * Z);
+            for (int c = 0; c < a.mcol; ++c) {
+              const int lo = g[c * Z + lr], hi = g[c * Z + lr + L];
+              const uint32_t p = ((uint32_t)(lo & 0xff) << 8) | ((uint32_t)(hi & 0xff) << 24);
+              wbase[c * a.col_dwords + pos] = p;
+              wbase[c * a.col_dwords + pos + 64] = (p >> 16) | (p << 16);
+            }
           }
         }
       }
@@ -275,11 +304,32 @@ k_pair_i8(const PairArgs a) {
     const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;
     const bool finish = active && (!cwfail || iter == a.max_iter);   // LDPCdecode.cpp:192-197
     if (finish && real) {
-      int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
-      for (int c = 0; c < a.mcol; ++c) {
-        const uint32_t p = wbase[c * a.col_dwords + pos];
-        g[c * Z + lr] = (int8_t)(p >> 8);
-        g[c * Z + lr + L] = (int8_t)(p >> 24);
+      if constexpr (FZ) {
+        // columns in groups of eight: the LDS reads first, then the byte stores at
+        // immediate offsets
+        const uint32_t* wp = wbase + pos;
+        uint8_t* g = reinterpret_cast<uint8_t*>(a.post_out) + cw * (long long)(SH::mc * ZC) + lr;
+        static_assert(SH::mc % 8 == 0, "column groups of eight");
+#pragma unroll
+        for (int c0 = 0; c0 < SH::mc; c0 += 8) {
+          uint32_t v[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) v[k] = wp[(c0 + k) * 128];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const uint32_t x = v[k] >> 8;
+            g[(c0 + k) * ZC] = (uint8_t)x;
+            g[(c0 + k) * ZC + ZC / 2] = (uint8_t)(x >> 16);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+        int8_t* g = a.post_out + cw * (long long)(a.mcol * Z);
+        for (int c = 0; c < a.mcol; ++c) {
+          const uint32_t p = wbase[c * a.col_dwords + pos];
+          g[c * Z + lr] = (int8_t)(p >> 8);
+          g[c * Z + lr + L] = (int8_t)(p >> 24);
+        }
       }
       if (a.c2v) {
         // opaque row offset: keeps the NE store offsets from being hoisted out of the
@@ -324,8 +374,8 @@ int pair_i8_shape(const DecoderDev& d) {
   return pair_shape_index(h.mrow, h.row_weights.data());
 }
 
-template <class SH>
-static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
+template <class SH, int ZC>
+static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   PairArgs a = in;
   if (a.num_cw == 0) return hipSuccess;
   const size_t lds = (size_t)a.waves * a.wave_dwords * 4;
@@ -346,11 +396,11 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
       // the twin addressing (byte address xor 256) needs the waves' columns at multiples
       // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
       hipFuncAttributes fa;
-      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH>) != hipSuccess || fa.sharedSizeBytes != 0)
+      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH, ZC>) != hipSuccess || fa.sharedSizeBytes != 0)
         return hipErrorInvalidDeviceFunction;
       int cus = 0, per_cu = 0;
       if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH>, 64 * a.waves, lds) != hipSuccess)
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH, ZC>, 64 * a.waves, lds) != hipSuccess)
         return hipErrorInvalidValue;
       resident_by_dev[dev] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
     }
@@ -362,8 +412,16 @@ static hipError_t launch_shape(const PairArgs& in, hipStream_t st) {
   if (blocks > cap_blocks) blocks = cap_blocks;
   const long long waves = blocks * a.waves;
   a.chunk = (a.num_cw + waves - 1) / waves;
-  hipLaunchKernelGGL((k_pair_i8<SH>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
+  hipLaunchKernelGGL((k_pair_i8<SH, ZC>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
   return hipGetLastError();
+}
+
+template <class SH>
+static hipError_t launch_shape(const PairArgs& a, hipStream_t st) {
+  if constexpr (SH::zc != 0) {
+    if (a.z == SH::zc && a.mcol == SH::mc && a.col_dwords == 128) return launch_shape_z<SH, SH::zc>(a, st);
+  }
+  return launch_shape_z<SH, 0>(a, st);
 }
 
 // Host-side layout of the pair kernel for decoder `d` (tables in d.h).

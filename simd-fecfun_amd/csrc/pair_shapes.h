@@ -6,46 +6,57 @@ namespace fecsim {
 
 struct PairShape0 {  // 11ad 1/2
   static constexpr int mrow = 8, ne = 52, maxw = 8;
+  static constexpr int zc = 42, mc = 16;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[8] = {5, 6, 6, 6, 7, 7, 7, 8};
 };
 struct PairShape1 {  // 11ad 13/16
   static constexpr int mrow = 3, ne = 45, maxw = 16;
+  static constexpr int zc = 42, mc = 16;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[3] = {14, 15, 16};
 };
 struct PairShape2 {  // 11ad 3/4
   static constexpr int mrow = 4, ne = 56, maxw = 15;
+  static constexpr int zc = 42, mc = 16;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[4] = {13, 14, 14, 15};
 };
 struct PairShape3 {  // 11ad 5/8
   static constexpr int mrow = 6, ne = 50, maxw = 10;
+  static constexpr int zc = 42, mc = 16;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[6] = {10, 10, 7, 8, 8, 7};
 };
 struct PairShape4 {  // 11n 1/2 z54
   static constexpr int mrow = 12, ne = 86, maxw = 8;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[12] = {7, 8, 7, 7, 7, 7, 7, 8, 7, 7, 7, 7};
 };
 struct PairShape5 {  // 11n 2/3 z54
   static constexpr int mrow = 8, ne = 88, maxw = 11;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[8] = {11, 11, 11, 11, 11, 11, 11, 11};
 };
 struct PairShape6 {  // 11n 3/4 z54
   static constexpr int mrow = 6, ne = 88, maxw = 15;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[6] = {15, 14, 15, 15, 15, 14};
 };
 struct PairShape7 {  // 11n 1/2 z27
   static constexpr int mrow = 12, ne = 88, maxw = 8;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[12] = {7, 8, 7, 7, 7, 8, 7, 7, 8, 7, 8, 7};
 };
 struct PairShape8 {  // 16e 1/2
   static constexpr int mrow = 12, ne = 76, maxw = 7;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[12] = {6, 7, 7, 6, 6, 7, 6, 6, 7, 6, 6, 6};
 };
 struct PairShape9 {  // 16e 2/3A
   static constexpr int mrow = 8, ne = 80, maxw = 10;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[8] = {10, 10, 10, 10, 10, 10, 10, 10};
 };
 struct PairShape10 {  // ITU G.hn 1/2 z14
   static constexpr int mrow = 12, ne = 77, maxw = 7;
+  static constexpr int zc = 0, mc = 0;  // Z and columns of the fixed-Z kernel (0: none)
   static constexpr int w[12] = {5, 7, 7, 6, 7, 6, 6, 6, 7, 6, 7, 7};
 };
 

@@ -11,6 +11,10 @@ import os
 
 from . import codes
 
+# codes whose lifting size is fixed by their standard: their shape also gets a kernel with
+# Z and the column count compiled in (loads and stores of a codeword at immediate offsets);
+# other lifting sizes (num_par > 1, 802.16e's sweep) take the run-time-Z kernel
+FIXED_Z_SETS = ["11ad 1/2", "11ad 13/16", "11ad 3/4", "11ad 5/8"]
 # codes whose lifting size can be even and <= 64 (pair kernel: Z/2 <= 32 banks)
 PAIR_SETS = ["11ad 1/2", "11ad 13/16", "11ad 3/4", "11ad 5/8", "11n 1/2 z54", "11n 2/3 z54", "11n 3/4 z54",
              "11n 1/2 z27", "16e 1/2", "16e 2/3A", "ITU G.hn 1/2 z14"]
@@ -18,12 +22,14 @@ MAX_EDGES = 90
 
 
 def shapes():
+    """[(row weights, matrix set, (Z, mcol) compiled in or (0, 0))]"""
     out = []
     for ms in PAIR_SETS:
-        hc, _, _ = codes.get_hmatrix(ms)
+        hc, z, _ = codes.get_hmatrix(ms)
         w = tuple(int((r >= 0).sum()) for r in hc)
-        if sum(w) <= MAX_EDGES and w not in [s for s, _ in out]:
-            out.append((w, ms))
+        if sum(w) <= MAX_EDGES and w not in [s for s, _, _ in out]:
+            zc = (z, hc.shape[1]) if ms in FIXED_Z_SETS and z % 2 == 0 else (0, 0)
+            out.append((w, ms, zc))
     return out
 
 
@@ -31,13 +37,14 @@ def emit(csrc_dir: str) -> None:
     sh = shapes()
     h = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.", "#pragma once", "#include <stdint.h>", "",
          "namespace fecsim {", ""]
-    for i, (w, ms) in enumerate(sh):
+    for i, (w, ms, (zc, mc)) in enumerate(sh):
         h.append("struct PairShape%d {  // %s" % (i, ms))
         h.append("  static constexpr int mrow = %d, ne = %d, maxw = %d;" % (len(w), sum(w), max(w)))
+        h.append("  static constexpr int zc = %d, mc = %d;  // Z and columns of the fixed-Z kernel (0: none)" % (zc, mc))
         h.append("  static constexpr int w[%d] = {%s};" % (len(w), ", ".join(map(str, w))))
         h.append("};")
     h += ["", "inline int pair_shape_index(int mrow, const int32_t* w) {"]
-    for i, (w, _) in enumerate(sh):
+    for i, (w, _, _) in enumerate(sh):
         cond = " && ".join("w[%d] == %d" % (k, x) for k, x in enumerate(w))
         h.append("  if (mrow == %d && %s) return %d;" % (len(w), cond, i))
     h += ["  return -1;", "}", "", "struct PairArgs;",
