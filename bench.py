@@ -356,6 +356,26 @@ def main():
         "learned_offset_gain_db": None,
     }
     if not args.no_learned:
+        # decode throughput with the committed learned offset table on the same batches
+        from fecsim.train import load_offsets
+        ltab, _ = load_offsets("11ad12_T10_int8")
+        ldec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+        ldec.set_offsets(ltab)
+        lo = torch.empty_like(llrs[0])
+        lit = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        for x in llrs:
+            ldec.decode(x, lo, iters=lit, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for x in llrs:
+            ldec.decode(x, lo, iters=lit, stream=stream)
+        torch.cuda.synchronize()
+        lt = time.perf_counter() - t0
+        res["learned_table_decode"] = {
+            "cw_per_s": ncw * len(SNRS) / lt, "info_gbit_s": ncw * len(SNRS) / lt * K_INFO / 1e9,
+            "kernel": "k_pair_i8<PairShape0> with the offset table" if ldec.kernel == _L.KERNEL_PAIR else "generic",
+            "note": "one pass over the five SNR batches with the learned per-(iteration, layer) offsets"}
+        del lo
         print("bench: learned offsets", file=sys.stderr, flush=True)
         lg = learned_gain(cs, henc, args.learned_ncw)
         res["learned_offset_gain_db"] = lg["gain_db"]

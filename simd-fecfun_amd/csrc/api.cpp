@@ -260,7 +260,7 @@ int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
 // `aligned`: the buffers are 16-byte aligned (the code kernel moves 16-byte chunks)
 static int dense_kernel(const DecoderDev& d, bool aligned) {
   if (d.kernel == FECSIM_KERNEL_GENERIC) return FECSIM_KERNEL_GENERIC;
-  if (d.pair_shape >= 0 && !d.d_beta_tab) return FECSIM_KERNEL_PAIR;
+  if (d.pair_shape >= 0 && (!d.d_beta_tab || d.d_beta_tab8)) return FECSIM_KERNEL_PAIR;
   if (d.code_tab >= 0 && aligned && (!d.d_beta_tab || d.beta_tab_code_ok)) return FECSIM_KERNEL_CODE;
   return FECSIM_KERNEL_GENERIC;
 }
@@ -277,7 +277,9 @@ int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
   DecoderDev& dv = d->d;
   if (!beta) {
     if (dv.d_beta_tab) (void)hipFree(dv.d_beta_tab);
+    if (dv.d_beta_tab8) (void)hipFree(dv.d_beta_tab8);
     dv.d_beta_tab = nullptr;
+    dv.d_beta_tab8 = nullptr;
     return FECSIM_OK;
   }
   const size_t n = (size_t)dv.params.max_iter * dv.h.mrow;
@@ -292,6 +294,21 @@ int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
     return hip_fail(he, "offset table to device");
   }
   dv.d_beta_tab = p;
+  // the row-pair kernel's int8 copy (its entries are the int8 offsets themselves)
+  if (dv.beta_tab_code_ok && pair_i8_table_ok(dv, n)) {
+    std::vector<int8_t> t8(tab.begin(), tab.end());
+    int8_t* q = dv.d_beta_tab8;
+    he = q ? hipSuccess : hipMalloc((void**)&q, n);
+    if (he == hipSuccess) he = hipMemcpy(q, t8.data(), n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+      if (q && !dv.d_beta_tab8) (void)hipFree(q);
+      return hip_fail(he, "offset table to device");
+    }
+    dv.d_beta_tab8 = q;
+  } else if (dv.d_beta_tab8) {
+    (void)hipFree(dv.d_beta_tab8);
+    dv.d_beta_tab8 = nullptr;
+  }
   return FECSIM_OK;
 }
 
@@ -310,6 +327,7 @@ void fecsim_decoder_free(fecsim_decoder* d) {
   if (d->d.d_pair_adr) (void)hipFree(d->d.d_pair_adr);
   if (d->d.d_c2v_scratch) (void)hipFree(d->d.d_c2v_scratch);
   if (d->d.d_beta_tab) (void)hipFree(d->d.d_beta_tab);
+  if (d->d.d_beta_tab8) (void)hipFree(d->d.d_beta_tab8);
   delete d;
 }
 

@@ -211,9 +211,13 @@ struct DecoderDev {
   void* d_beta_tab = nullptr;     // per-(iteration, layer) offsets: long long (int kinds) or double
   int code_tab = -1;              // code compiled into the code-specialized int8 kernel, -1 if none
   bool beta_tab_code_ok = false;  // the offset table suits that kernel (every entry 1..127)
+  int8_t* d_beta_tab8 = nullptr;  // the same table as int8 for the row-pair kernel, when it suits it
+                                  // (entries 1..127, fixed-Z shape, at most 4096 entries)
 };
 
 int pair_i8_shape(const DecoderDev& d);
+// the row-pair kernel can run decoder d with an offset table of n entries
+bool pair_i8_table_ok(const DecoderDev& d, size_t n);
 void pair_layout(const HdecHost& h, PairLayout* pl);
 hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 // code-specialized int8 kernel (one codeword per workgroup, code compiled in): the code's

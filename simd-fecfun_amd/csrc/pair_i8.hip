@@ -86,6 +86,7 @@ struct PairArgs {
   long long chunk;         // codewords per wave (persistent grid)
   int max_blocks;          // resident-grid size bound (0: one block per cw_per_wave*waves codewords)
   int beta, cap;           // int8 beta (1..127) and cap (0..127)
+  const int8_t* beta_tab;  // [max_iter][mrow] per-(iteration, layer) int8 offsets (1..127) or null
   const int8_t* llr_in;    // dense [num_cw][mcol*z]
   int8_t* post_out;
   uint32_t* iters;
@@ -104,7 +105,9 @@ constexpr int pair_waves_per_eu() {
 // With ZC the codeword's loads and stores take immediate offsets from one base address
 // and its column loops unroll (a refill or write-out costs 1-2 VALU per column instead of
 // the 64-bit address arithmetic of each byte).
-template <class SH, int ZC>
+// BT: per-(iteration, layer) offsets from a.beta_tab, copied behind the waves' columns in
+// LDS; each lane reads its slot's offset once per layer.
+template <class SH, int ZC, bool BT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<SH>())))
 k_pair_i8(const PairArgs a) {
   constexpr int NE = SH::ne;
@@ -148,6 +151,14 @@ k_pair_i8(const PairArgs a) {
   const ushort2v capb2 = ushort2v{(unsigned short)capb_i, (unsigned short)capb_i};
   const unsigned long long gmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
   unsigned long long my_iters = 0;   // summed by lane 0 of each slot
+  // offset table: [max_iter][mrow] bytes behind the waves' columns
+  const uint8_t* btab = reinterpret_cast<const uint8_t*>(smem + (size_t)a.waves * a.wave_dwords);
+  if constexpr (BT) {
+    uint8_t* bt = const_cast<uint8_t*>(btab);
+    const int nb = (int)a.max_iter * SH::mrow;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) bt[i] = (uint8_t)a.beta_tab[i];
+    __syncthreads();
+  }
 
   for (;;) {
     // ---- refill: idle slots take the next codewords of the wave's range ----
@@ -206,10 +217,20 @@ k_pair_i8(const PairArgs a) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) asm volatile("" : "+v"(adr[e]));   // unpack at each use, not hoisted
     int e0 = 0;
+    // this slot's offsets in iteration `iter` (idle slots keep counting iterations: clamped
+    // so that their reads stay inside the table)
+    const uint8_t* bt_it = btab + (iter < a.max_iter ? iter : a.max_iter - 1u) * SH::mrow;
 #pragma unroll
     for (int l = 0; l < SH::mrow; ++l) {
       constexpr int MAXW = SH::maxw;
       const int w = SH::w[l];
+      ushort2v beta2l = beta2, capb2l = capb2;
+      if constexpr (BT) {   // the layer's offset b: beta2 = b << 8, capb2 = min(cap + b, 128) << 8
+        const uint32_t b = bt_it[l];
+        const uint32_t cb = b + (uint32_t)a.cap;
+        beta2l = as_us(b * 0x01000100u);
+        capb2l = as_us((cb < 128u ? cb : 128u) * 0x01000100u);
+      }
       // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128).  With beta >= 1,
       // A_j = max(|Q_j| - beta, 0) and the capped exclusive minimum is, per row,
       //   mv1 = min(cap, max(m1, beta) - beta),  mv2 = min(max(m2, beta), cap + beta) - beta,
@@ -239,8 +260,8 @@ k_pair_i8(const PairArgs a) {
           m1 = umin(m1, av);
         }
       }
-      const short2v mv1 = smin(cap2, as_s(as_u(umax(m1, beta2) - beta2)));
-      const short2v mv2 = as_s(as_u(umin(umax(m2, beta2), capb2) - beta2));
+      const short2v mv1 = smin(cap2, as_s(as_u(umax(m1, beta2l) - beta2l)));
+      const short2v mv2 = as_s(as_u(umin(umax(m2, beta2l), capb2l) - beta2l));
       uint32_t sumb_u = as_u(m1 + as_us(as_u(mv2)));
       asm volatile("" : "+v"(sumb_u));   // one subtraction per edge, not re-associated into two
       const short2v sumb = as_s(sumb_u);
@@ -360,6 +381,10 @@ k_pair_i8(const PairArgs a) {
 
 // ---------------------------------------------------------------------------
 
+bool pair_i8_table_ok(const DecoderDev& d, size_t n) {
+  return d.pair_shape >= 0 && n <= 4096 && pair_shape_fixed_z(d.pair_shape, d.h.z, d.h.mcol);
+}
+
 int pair_i8_shape(const DecoderDev& d) {
   const HdecHost& h = d.h;
   if (d.elem != FECSIM_I8 || d.mode != FECSIM_MIN) return -1;
@@ -374,11 +399,11 @@ int pair_i8_shape(const DecoderDev& d) {
   return pair_shape_index(h.mrow, h.row_weights.data());
 }
 
-template <class SH, int ZC>
+template <class SH, int ZC, bool BT>
 static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   PairArgs a = in;
   if (a.num_cw == 0) return hipSuccess;
-  const size_t lds = (size_t)a.waves * a.wave_dwords * 4;
+  const size_t lds = (size_t)a.waves * a.wave_dwords * 4 + (BT ? ((size_t)a.max_iter * SH::mrow + 3) / 4 * 4 : 0);
   // grid of 8x the resident blocks, each wave working through a contiguous chunk of
   // codewords with per-slot refill: chunks of ~30 codewords per wave keep the spread of
   // iteration counts from leaving a long tail (measured on 11ad 1/2: 2x -> 8x is +5.6%,
@@ -396,11 +421,11 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
       // the twin addressing (byte address xor 256) needs the waves' columns at multiples
       // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
       hipFuncAttributes fa;
-      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH, ZC>) != hipSuccess || fa.sharedSizeBytes != 0)
+      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH, ZC, BT>) != hipSuccess || fa.sharedSizeBytes != 0)
         return hipErrorInvalidDeviceFunction;
       int cus = 0, per_cu = 0;
       if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH, ZC>, 64 * a.waves, lds) != hipSuccess)
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH, ZC, BT>, 64 * a.waves, lds) != hipSuccess)
         return hipErrorInvalidValue;
       resident_by_dev[dev] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
     }
@@ -412,16 +437,18 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   if (blocks > cap_blocks) blocks = cap_blocks;
   const long long waves = blocks * a.waves;
   a.chunk = (a.num_cw + waves - 1) / waves;
-  hipLaunchKernelGGL((k_pair_i8<SH, ZC>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
+  hipLaunchKernelGGL((k_pair_i8<SH, ZC, BT>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
   return hipGetLastError();
 }
 
 template <class SH>
 static hipError_t launch_shape(const PairArgs& a, hipStream_t st) {
   if constexpr (SH::zc != 0) {
-    if (a.z == SH::zc && a.mcol == SH::mc && a.col_dwords == 128) return launch_shape_z<SH, SH::zc>(a, st);
+    if (a.z == SH::zc && a.mcol == SH::mc && a.col_dwords == 128)
+      return a.beta_tab ? launch_shape_z<SH, SH::zc, true>(a, st) : launch_shape_z<SH, SH::zc, false>(a, st);
   }
-  return launch_shape_z<SH, 0>(a, st);
+  if (a.beta_tab) return hipErrorInvalidValue;   // tables run on the fixed-Z instances only
+  return launch_shape_z<SH, 0, false>(a, st);
 }
 
 // Host-side layout of the pair kernel for decoder `d` (tables in d.h).
@@ -462,6 +489,7 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   a.max_iter = d.params.max_iter;
   a.beta = (int)(int8_t)(int)d.params.beta_offset;
   a.cap = (int)(int8_t)(int)d.params.max_row_metric;
+  a.beta_tab = d.d_beta_tab8;
   a.llr_in = reinterpret_cast<const int8_t*>(ka.metrics_in);
   a.post_out = reinterpret_cast<int8_t*>(ka.metrics_out);
   a.iters = ka.iters;

@@ -75,6 +75,17 @@ inline int pair_shape_index(int mrow, const int32_t* w) {
   return -1;
 }
 
+// the shape has a kernel with lifting size z and mcol columns compiled in
+inline bool pair_shape_fixed_z(int shape, int z, int mcol) {
+  switch (shape) {
+  case 0: return z == 42 && mcol == 16;
+  case 1: return z == 42 && mcol == 16;
+  case 2: return z == 42 && mcol == 16;
+  case 3: return z == 42 && mcol == 16;
+  default: return false;
+  }
+}
+
 struct PairArgs;
 hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st);
 

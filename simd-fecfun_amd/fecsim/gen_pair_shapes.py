@@ -47,7 +47,13 @@ def emit(csrc_dir: str) -> None:
     for i, (w, _, _) in enumerate(sh):
         cond = " && ".join("w[%d] == %d" % (k, x) for k, x in enumerate(w))
         h.append("  if (mrow == %d && %s) return %d;" % (len(w), cond, i))
-    h += ["  return -1;", "}", "", "struct PairArgs;",
+    h += ["  return -1;", "}", "",
+          "// the shape has a kernel with lifting size z and mcol columns compiled in",
+          "inline bool pair_shape_fixed_z(int shape, int z, int mcol) {", "  switch (shape) {"]
+    for i, (_, _, (zc, mc)) in enumerate(sh):
+        if zc:
+            h.append("  case %d: return z == %d && mcol == %d;" % (i, zc, mc))
+    h += ["  default: return false;", "  }", "}", "", "struct PairArgs;",
           "hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st);", "", "}  // namespace fecsim", ""]
     d = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
          "// included inside namespace fecsim by pair_i8.hip after FECSIM_PAIR_LAUNCH is defined",

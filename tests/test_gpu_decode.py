@@ -460,3 +460,78 @@ def test_gpu_global_c2v_codes_match_oracle(case, cuda):
     assert tot == ref[3]
     ce, _ = refsim.count_errors(cs, bits, post, elem=elem)
     _say("  iterations", iters.mean(), "cw errors", ce, "of", ncw)
+
+
+@pytest.mark.parametrize("case", [(-0.5, 10, 31.0, 600), (0.0, 6, 9.0, 600), (-1.0, 10, 127.0, 300),
+                                  (-0.5, 20, 0.5, 300)],
+                         ids=lambda c: "snr%g-it%d-cap%g-n%d" % c)
+def test_gpu_pair_kernel_offset_tables(case, cuda):
+    """Per-(iteration, layer) offset tables on the row-pair kernel (802.11ad rate 1/2,
+    entries 1..4): posteriors, C2V and iteration counts equal the oracle's with the same
+    table; the same decoder with a constant table equals the plain decoder; a table with
+    a 0 entry or a lifted Z falls back to the generic kernel."""
+    import torch
+    from fecsim import lib as L
+    snr, mi, cap, ncw = case
+    _say("pair kernel table", case)
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(ncw)
+    llr, _ = rs.metrics(wgn, bits, snr)
+    rng = np.random.default_rng(int(mi * 7 + cap))
+    llr[0] = rng.integers(-128, 128, cs.n)
+    llr[1] = np.where(rng.random(cs.n) < 0.5, -128, 127)
+    tab = rng.integers(1, 5, (mi, cs.dec.mrow)).astype(np.float32)
+    want = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab)
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, cap, 1.0)
+    d.set_offsets(tab)
+    assert d.kernel == L.KERNEL_PAIR
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+    c2v = torch.zeros(ncw * cs.dec.total_hc_ones * cs.z, dtype=torch.int8, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    d.decode(x, out, iters=it, c2v=c2v, tot_iter=tot)
+    torch.cuda.synchronize()
+    assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
+    assert np.array_equal(out.cpu().numpy(), want[0])
+    assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want[1])
+    assert int(tot.item()) == want[3]
+    # a constant table of 1 is the plain beta-1 decoder
+    d.set_offsets(np.ones((mi, cs.dec.mrow), np.float32))
+    assert d.kernel == L.KERNEL_PAIR
+    plain = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0)
+    d.decode(x, out, iters=it)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), plain[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), plain[2])
+    # an entry of 0 (the abs(-128) wrap path): generic kernel, same results as the oracle
+    tab0 = tab.copy()
+    tab0[0, 3] = 0.0
+    d.set_offsets(tab0)
+    assert d.kernel == L.KERNEL_GENERIC
+    want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab0)
+    d.decode(x, out, iters=it)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), want0[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), want0[2])
+
+
+def test_gpu_pair_kernel_table_lifted_z_falls_back(cuda):
+    """num_par = 2 lifts 11ad 1/2 to Z = 84 (no row-pair kernel); with a table the generic
+    kernel decodes it, equal to the oracle."""
+    import torch
+    from fecsim import lib as L
+    cs = codes.gen_structs("11ad 1/2", num_par=2, simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    wgn, bits = rs.inner_loop(64)
+    llr, _ = rs.metrics(wgn, bits, -0.5)
+    tab = np.random.default_rng(4).integers(1, 4, (10, cs.dec.mrow)).astype(np.float32)
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+    d.set_offsets(tab)
+    assert d.kernel == L.KERNEL_GENERIC
+    want = O.decode_ideal(cs.dec, llr, max_iter=10, beta=1.0, beta_table=tab)
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    it = torch.zeros(64, dtype=torch.int32, device="cuda")
+    d.decode(x, out, iters=it)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), want[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
