@@ -147,6 +147,7 @@ struct TrainArgs {
   int mrow, z, n, e, n_pad;  // n_pad: LDS elements per posterior array
   int nedge, tab_bytes;      // total_hc_ones; LDS bytes of the [edges | layer_w] copy in front
   int iters;                 // T
+  int maxw;                  // largest row weight
   int waves;                 // codewords (waves) per block
   int is_double, global_p;   // element type; posteriors in global scratch instead of LDS
   double cap;
@@ -155,9 +156,11 @@ struct TrainArgs {
   const void* alpha;         // [T][mrow]
   const void* beta;          // [T][mrow]
   void* post;                // [batch][n]
-  void* ckP;                 // [batch][T][n] posteriors at the start of every iteration
-  void* ckC;                 // [batch][T+1][e] C2V at the start of every iteration (+ final)
-  void* qs;                  // [batch][e] V2C messages of the iteration being reversed
+  int seg;                   // K: checkpoint every K iterations (ceil(sqrt(T)))
+  void* ckP;                 // [batch][ceil(T/K)][n] posteriors at the start of iterations 0, K, 2K, ...
+  void* ckC;                 // [batch][ceil(T/K)][e] C2V at those iterations
+  void* wc;                  // [batch][e] working C2V
+  void* qs;                  // [batch][K][e] V2C messages of the segment being reversed
   void* gc;                  // [batch][e] adjoint of the C2V messages
   void* partial;             // [batch][T][mrow][2] per-codeword parameter gradients
   void* gscratch;            // [batch][2][n] posteriors + adjoints when global_p

@@ -8,15 +8,21 @@
 // With alpha = 1 and a constant beta it equals the reference decoder on codewords
 // that need all T iterations.
 //
-// Adjoint: reverse sweep over iterations; iteration t restarts from its checkpoint
-// (posteriors P_t and C2V C_t stored by the forward), recomputes its V2C messages
-// q (kept in a scratch array), then walks the layers backwards:
+// Memory: the forward keeps checkpoints only every K = ceil(sqrt(T)) iterations
+// (posteriors P and C2V C at the start of iterations 0, K, 2K, ...) and one working C2V
+// array.  The adjoint takes the segments [sK, sK + K) in reverse: it restarts from the
+// segment's checkpoint and runs the segment's iterations again, keeping their V2C
+// messages q (the only values the reverse sweep needs), then sweeps those iterations
+// backwards, layer by layer:
 //   g_out_j = gP[v_j] + gC[e_j]           (C2V_j feeds the posterior and iteration t+1)
 //   g_m_j   = sign_j * g_out_j,  g_alpha += g_m_j * min(cap, x_j),  g_A[src_j] += alpha * g_m_j (x_j < cap)
 //   g_q_j   = gP[v_j] + [|q_j| > beta] * sign(q_j) * g_A_j,  g_beta -= [|q_j| > beta] * g_A_j
 //   gP[v_j] <- g_q_j,  gC[e_j] <- -g_q_j    (gradient w.r.t. the C2V entering iteration t)
-// where x_j is the exclusive minimum and src_j the edge it came from.  Parameter
-// gradients are written per codeword and summed by k_train_sum (deterministic).
+// where x_j is the exclusive minimum and src_j the edge it came from: the row's argmin
+// for every other edge, its second minimum for the argmin itself.  Per codeword and
+// iteration this stores (N + E)/K + E values instead of N + E, and recomputes each
+// iteration once.  Parameter gradients are written per codeword and summed by
+// k_train_sum (deterministic).
 //
 // One wave per codeword, lane owns rows lane, lane+64, ... of every layer.  Every block
 // first copies the code's edge words and layer weights into the front of its LDS.
@@ -54,17 +60,18 @@ __device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t 
   return v;
 }
 
-// one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q
-template <typename F>
+// one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q.
+// MAXW bounds the row weight w (the unrolled loops stop there).
+template <typename F, int MAXW>
 __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __restrict__ edges, F* P, const F* Cold,
                                         F* Cnew, F* Qs, int e0, int w, int r, F al, F be, F cap) {
   const int Z = a.z;
-  F q[kTrainMaxW];
-  int vi[kTrainMaxW];
+  F q[MAXW];
+  int vi[MAXW];
   F m1 = F(INFINITY), m2 = F(INFINITY);
   int im = -1, sg = 0;
 #pragma unroll
-  for (int j = 0; j < kTrainMaxW; ++j) {
+  for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
       vi[j] = edge_var(edges[e0 + j], r, Z);
       const int ce = (e0 + j) * Z + r;
@@ -78,7 +85,7 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
     }
   }
 #pragma unroll
-  for (int j = 0; j < kTrainMaxW; ++j) {
+  for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
       const F x = (j == im) ? m2 : m1;
       const F m = al * (cap < x ? cap : x);
@@ -90,6 +97,64 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
   }
 }
 
+// one check row of the reverse sweep (see the head of this file); adds the row's
+// parameter gradients to *ga, *gb
+template <typename F, int MAXW>
+__device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* gP, F* gC, const F* Qs, int e0, int w,
+                                        int r, int Z, F al, F be, F cap, F* ga, F* gb) {
+  F q[MAXW], gq[MAXW];
+  int vi[MAXW];
+  F m1 = F(INFINITY), m2 = F(INFINITY);
+  int im = -1, im2 = -1, sgn = 0;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      vi[j] = edge_var(edges[e0 + j], r, Z);
+      q[j] = Qs[(e0 + j) * Z + r];
+      const F x = fabs(q[j]) - be;
+      const F A = x > F(0) ? x : F(0);
+      sgn ^= sbit(q[j]);
+      if (A < m1) { m2 = m1; im2 = im; m1 = A; im = j; }
+      else if (A < m2) { m2 = A; im2 = j; }
+    }
+  }
+  // gradient reaching A_k: every edge other than the argmin took its value from the
+  // argmin (summed in edge order), the argmin took it from the second minimum
+  F g_im = F(0), g_im2 = F(0), gaa = *ga, gbb = *gb;
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const int ce = (e0 + j) * Z + r;
+      const F x = (j == im) ? m2 : m1;
+      const bool capped = cap < x;
+      const F mc = capped ? cap : x;
+      const F s = (sgn ^ sbit(q[j])) ? F(-1) : F(1);
+      const F gv = gP[vi[j]];
+      const F gm = s * (gv + gC[ce]);
+      gq[j] = gv;
+      gaa += gm * mc;
+      if (!capped) {
+        if (j == im) g_im2 += al * gm;
+        else g_im += al * gm;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MAXW; ++j) {
+    if (j < w) {
+      const F gA = (j == im) ? g_im : ((j == im2) ? g_im2 : F(0));
+      if (fabs(q[j]) - be > F(0)) {
+        gq[j] += sbit(q[j]) ? -gA : gA;
+        gbb -= gA;
+      }
+      gP[vi[j]] = gq[j];
+      gC[(e0 + j) * Z + r] = -gq[j];
+    }
+  }
+  *ga = gaa;
+  *gb = gbb;
+}
+
 template <typename F>
 __device__ __forceinline__ F wave_sum(F v) {
 #pragma unroll
@@ -97,7 +162,7 @@ __device__ __forceinline__ F wave_sum(F v) {
   return v;
 }
 
-template <typename F, bool GP>
+template <typename F, bool GP, int MAXW>
 __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -112,24 +177,31 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   F* P = tv.P;
   const int N = a.n, E = a.e, Z = a.z, T = a.iters;
   const F* llr = reinterpret_cast<const F*>(a.llr) + cw * (uint64_t)N;
-  F* ckP = reinterpret_cast<F*>(a.ckP) + cw * (uint64_t)T * N;
-  F* ckC = reinterpret_cast<F*>(a.ckC) + cw * (uint64_t)(T + 1) * E;
+  const int K = a.seg, NS = (T + K - 1) / K;
+  F* ckP = reinterpret_cast<F*>(a.ckP) + cw * (uint64_t)NS * N;
+  F* ckC = reinterpret_cast<F*>(a.ckC) + cw * (uint64_t)NS * E;
+  F* C = reinterpret_cast<F*>(a.wc) + cw * (uint64_t)E;   // working C2V, updated in place
   const F* alpha = reinterpret_cast<const F*>(a.alpha);
   const F* beta = reinterpret_cast<const F*>(a.beta);
   const F cap = (F)a.cap;
   for (int i = lane; i < N; i += 64) P[i] = llr[i];
-  for (int i = lane; i < E; i += 64) ckC[i] = F(0);
+  for (int i = lane; i < E; i += 64) C[i] = F(0);
   wave_sync();
+  const int tlast = (NS - 1) * K;   // the adjoint starts with this segment: its q are kept now
+  F* Qseg = reinterpret_cast<F*>(a.qs) + cw * (uint64_t)K * E;
   for (int t = 0; t < T; ++t) {
-    F* cp = ckP + (uint64_t)t * N;
-    for (int i = lane; i < N; i += 64) cp[i] = P[i];
-    const F* Cold = ckC + (uint64_t)t * E;
-    F* Cnew = ckC + (uint64_t)(t + 1) * E;
+    F* Qs = t >= tlast ? Qseg + (uint64_t)(t - tlast) * E : (F*)nullptr;
+    if (t % K == 0) {   // checkpoint at the start of segment t / K
+      F* cp = ckP + (uint64_t)(t / K) * N;
+      F* cc = ckC + (uint64_t)(t / K) * E;
+      for (int i = lane; i < N; i += 64) cp[i] = P[i];
+      for (int i = lane; i < E; i += 64) cc[i] = C[i];
+    }
     int e0 = 0;
     for (int l = 0; l < a.mrow; ++l) {
       const int w = layer_w[l];
       const F al = alpha[t * a.mrow + l], be = beta[t * a.mrow + l];
-      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, edges, P, Cold, Cnew, (F*)nullptr, e0, w, r, al, be, cap);
+      for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
       e0 += w;
       wave_sync();
     }
@@ -138,7 +210,7 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   for (int i = lane; i < N; i += 64) out[i] = P[i];
 }
 
-template <typename F, bool GP>
+template <typename F, bool GP, int MAXW>
 __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -153,9 +225,11 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   F* P = tv.P;
   F* gP = tv.gP;
   const int N = a.n, E = a.e, Z = a.z, T = a.iters, Lr = a.mrow;
-  const F* ckP = reinterpret_cast<const F*>(a.ckP) + cw * (uint64_t)T * N;
-  const F* ckC = reinterpret_cast<const F*>(a.ckC) + cw * (uint64_t)(T + 1) * E;
-  F* Qs = reinterpret_cast<F*>(a.qs) + cw * (uint64_t)E;
+  const int K = a.seg, NS = (T + K - 1) / K;
+  const F* ckP = reinterpret_cast<const F*>(a.ckP) + cw * (uint64_t)NS * N;
+  const F* ckC = reinterpret_cast<const F*>(a.ckC) + cw * (uint64_t)NS * E;
+  F* C = reinterpret_cast<F*>(a.wc) + cw * (uint64_t)E;
+  F* Qseg = reinterpret_cast<F*>(a.qs) + cw * (uint64_t)K * E;   // q of the segment's iterations
   F* gC = reinterpret_cast<F*>(a.gc) + cw * (uint64_t)E;
   F* part = reinterpret_cast<F*>(a.partial) + cw * (uint64_t)T * Lr * 2;
   const F* alpha = reinterpret_cast<const F*>(a.alpha);
@@ -165,80 +239,44 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   for (int i = lane; i < N; i += 64) gP[i] = gpost[i];
   for (int i = lane; i < E; i += 64) gC[i] = F(0);
   wave_sync();
-  for (int t = T - 1; t >= 0; --t) {
-    const F* cp = ckP + (uint64_t)t * N;
-    for (int i = lane; i < N; i += 64) P[i] = cp[i];
-    const F* Cold = ckC + (uint64_t)t * E;
-    wave_sync();
-    int e0 = 0;
-    for (int l = 0; l < Lr; ++l) {   // recompute, keeping q
-      const int w = layer_w[l];
-      const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
-      for (int r = lane; r < Z; r += 64) fwd_row<F>(a, edges, P, Cold, (F*)nullptr, Qs, e0, w, r, al, be, cap);
-      e0 += w;
+  for (int sg = NS - 1; sg >= 0; --sg) {
+    const int t0 = sg * K, t1 = (t0 + K < T) ? t0 + K : T;
+    // the segment's forward again, from its checkpoint, keeping every iteration's q (the
+    // last segment's q were kept by the forward)
+    if (sg < NS - 1) {
+      const F* cp = ckP + (uint64_t)sg * N;
+      const F* cc = ckC + (uint64_t)sg * E;
+      for (int i = lane; i < N; i += 64) P[i] = cp[i];
+      for (int i = lane; i < E; i += 64) C[i] = cc[i];
       wave_sync();
-    }
-    for (int l = Lr - 1; l >= 0; --l) {
-      const int w = layer_w[l];
-      e0 -= w;
-      const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
-      F ga = F(0), gb = F(0);
-      for (int r = lane; r < Z; r += 64) {
-        F q[kTrainMaxW], gA[kTrainMaxW], gq[kTrainMaxW];
-        int vi[kTrainMaxW];
-        F m1 = F(INFINITY), m2 = F(INFINITY);
-        int im = -1, im2 = -1, sg = 0;
-#pragma unroll
-        for (int j = 0; j < kTrainMaxW; ++j) {
-          gA[j] = F(0);
-          if (j < w) {
-            vi[j] = edge_var(edges[e0 + j], r, Z);
-            q[j] = Qs[(e0 + j) * Z + r];
-            const F x = fabs(q[j]) - be;
-            const F A = x > F(0) ? x : F(0);
-            sg ^= sbit(q[j]);
-            if (A < m1) { m2 = m1; im2 = im; m1 = A; im = j; }
-            else if (A < m2) { m2 = A; im2 = j; }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < kTrainMaxW; ++j) {
-          if (j < w) {
-            const int ce = (e0 + j) * Z + r;
-            const F x = (j == im) ? m2 : m1;
-            const int src = (j == im) ? im2 : im;
-            const bool capped = cap < x;
-            const F mc = capped ? cap : x;
-            const F s = (sg ^ sbit(q[j])) ? F(-1) : F(1);
-            const F gv = gP[vi[j]];
-            const F gm = s * (gv + gC[ce]);
-            gq[j] = gv;
-            ga += gm * mc;
-            if (!capped) {
-#pragma unroll
-              for (int k = 0; k < kTrainMaxW; ++k)
-                if (k == src) gA[k] += al * gm;
-            }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < kTrainMaxW; ++j) {
-          if (j < w) {
-            if (fabs(q[j]) - be > F(0)) {
-              gq[j] += sbit(q[j]) ? -gA[j] : gA[j];
-              gb -= gA[j];
-            }
-            gP[vi[j]] = gq[j];
-            gC[(e0 + j) * Z + r] = -gq[j];
-          }
+      for (int t = t0; t < t1; ++t) {
+        F* Qs = Qseg + (uint64_t)(t - t0) * E;
+        int e0 = 0;
+        for (int l = 0; l < Lr; ++l) {
+          const int w = layer_w[l];
+          const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
+          for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
+          e0 += w;
+          wave_sync();
         }
       }
-      wave_sync();
-      ga = wave_sum(ga);
-      gb = wave_sum(gb);
-      if (lane == 0) {
-        part[(t * Lr + l) * 2 + 0] = ga;
-        part[(t * Lr + l) * 2 + 1] = gb;
+    }
+    for (int t = t1 - 1; t >= t0; --t) {
+      const F* Qs = Qseg + (uint64_t)(t - t0) * E;
+      int e0 = a.nedge;
+      for (int l = Lr - 1; l >= 0; --l) {
+        const int w = layer_w[l];
+        e0 -= w;
+        const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
+        F ga = F(0), gb = F(0);
+        for (int r = lane; r < Z; r += 64) rev_row<F, MAXW>(edges, gP, gC, Qs, e0, w, r, Z, al, be, cap, &ga, &gb);
+        wave_sync();
+        ga = wave_sum(ga);
+        gb = wave_sum(gb);
+        if (lane == 0) {
+          part[(t * Lr + l) * 2 + 0] = ga;
+          part[(t * Lr + l) * 2 + 1] = gb;
+        }
       }
     }
   }
@@ -266,13 +304,22 @@ __global__ void __launch_bounds__(256) k_train_sum(const F* partial, uint32_t ba
   }
 }
 
-template <typename F, bool GP>
-hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
+template <typename F, bool GP, int MAXW>
+hipError_t launch_tw(const TrainArgs& a, int bwd, hipStream_t st) {
   const uint64_t blocks = (a.batch + a.waves - 1) / a.waves;
   const size_t lds = (size_t)a.tab_bytes + (GP ? 0 : (size_t)a.waves * 2 * a.n_pad * sizeof(F));
-  if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
-  else hipLaunchKernelGGL((k_train_fwd<F, GP>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
-  hipError_t e = hipGetLastError();
+  if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP, MAXW>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  else hipLaunchKernelGGL((k_train_fwd<F, GP, MAXW>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  return hipGetLastError();
+}
+
+// the kernels' row loops unroll to the smallest weight bound that holds the code's rows
+template <typename F, bool GP>
+hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
+  hipError_t e = a.maxw <= 8    ? launch_tw<F, GP, 8>(a, bwd, st)
+                 : a.maxw <= 16 ? launch_tw<F, GP, 16>(a, bwd, st)
+                 : a.maxw <= 24 ? launch_tw<F, GP, 24>(a, bwd, st)
+                                : launch_tw<F, GP, kTrainMaxW>(a, bwd, st);
   if (e != hipSuccess || !bwd) return e;
   const int per_cw = a.iters * a.mrow * 2;
   hipLaunchKernelGGL(k_train_sum<F>, dim3(per_cw), dim3(256), 0, st, reinterpret_cast<const F*>(a.partial),

@@ -16,8 +16,10 @@ struct fecsim_trainer {
   int waves = 4, global_p = 0, n_pad = 0, tab_bytes = 0;
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
+  int seg = 1;   // checkpoint every seg iterations
   void* ckP = nullptr;
   void* ckC = nullptr;
+  void* wc = nullptr;
   void* qs = nullptr;
   void* gc = nullptr;
   void* partial = nullptr;
@@ -37,6 +39,7 @@ void free_all(fecsim_trainer* t) {
   (void)hipFree(t->d_layer_w);
   (void)hipFree(t->ckP);
   (void)hipFree(t->ckC);
+  (void)hipFree(t->wc);
   (void)hipFree(t->qs);
   (void)hipFree(t->gc);
   (void)hipFree(t->partial);
@@ -56,13 +59,16 @@ TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
   a.nedge = t->h.total_hc_ones;
   a.tab_bytes = t->tab_bytes;
   a.iters = (int)t->p.iters;
+  a.maxw = h.max_row_weight;
   a.waves = t->waves;
   a.is_double = t->p.elem == FECSIM_F64;
   a.global_p = t->global_p;
   a.cap = (double)t->p.cap;
   a.batch = batch;
+  a.seg = t->seg;
   a.ckP = t->ckP;
   a.ckC = t->ckC;
+  a.wc = t->wc;
   a.qs = t->qs;
   a.gc = t->gc;
   a.partial = t->partial;
@@ -105,9 +111,15 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (he == hipSuccess) he = hipMalloc((void**)&t->d_layer_w, sizeof(int32_t) * h.mrow);
   if (he == hipSuccess) he = hipMemcpy(t->d_edges, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(t->d_layer_w, h.row_weights.data(), sizeof(int32_t) * h.mrow, hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMalloc(&t->ckP, B * T * N * eb);
-  if (he == hipSuccess) he = hipMalloc(&t->ckC, B * (T + 1) * E * eb);
-  if (he == hipSuccess) he = hipMalloc(&t->qs, B * E * eb);
+  // checkpoints every K = ceil(sqrt(T)) iterations; the adjoint recomputes one segment of K
+  size_t K = 1;
+  while (K * K < T) ++K;
+  t->seg = (int)K;
+  const size_t NS = (T + K - 1) / K;
+  if (he == hipSuccess) he = hipMalloc(&t->ckP, B * NS * N * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->ckC, B * NS * E * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->wc, B * E * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->qs, B * K * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->gc, B * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->partial, B * T * h.mrow * 2 * eb);
   if (he == hipSuccess && t->global_p) he = hipMalloc(&t->gscratch, B * 2 * N * eb);
