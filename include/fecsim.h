@@ -108,9 +108,9 @@ int fecsim_decoder_info(const fecsim_decoder *d, int *elem, int *mode, int *elem
  * beta_offset.  NULL restores the constant offset.  No reference counterpart: this is
  * how offsets learned with the trainable decoder below are run in the bit-exact
  * decoders; with every entry equal to beta_offset the results are the plain
- * decoder's.  A table whose int8 entries are all 1..127 runs on the row-pair kernel
- * (802.11ad shapes at their own Z, at most 4096 entries) and on the code-specialized
- * kernel; other tables run on the generic kernel.  Returns FECSIM_E_INVAL
+ * decoder's.  Tables run on the row-pair kernel (802.11ad shapes at their own Z, at
+ * most 4096 entries, int8 entries 0..127) and on the code-specialized kernel (entries
+ * 1..127); other tables run on the generic kernel.  Returns FECSIM_E_INVAL
  * while a queue decoder (fecsim_multi_create) or a simulation loop (fecsim_sim_create)
  * holds d: they copy its tables and kernel choice when they are made, so set offsets
  * before creating them (or after freeing them). */

@@ -294,8 +294,10 @@ int fecsim_decoder_set_offsets(fecsim_decoder* d, const float* beta) {
     return hip_fail(he, "offset table to device");
   }
   dv.d_beta_tab = p;
-  // the row-pair kernel's int8 copy (its entries are the int8 offsets themselves)
-  if (dv.beta_tab_code_ok && pair_i8_table_ok(dv, n)) {
+  // the row-pair kernel's int8 copy (its entries are the int8 offsets themselves, 0..127)
+  bool pair_ok = dv.elem == FECSIM_I8 && pair_i8_table_ok(dv, n);
+  for (long long b : tab) pair_ok = pair_ok && b >= 0 && b <= 127;
+  if (pair_ok) {
     std::vector<int8_t> t8(tab.begin(), tab.end());
     int8_t* q = dv.d_beta_tab8;
     he = q ? hipSuccess : hipMalloc((void**)&q, n);

@@ -215,7 +215,7 @@ struct DecoderDev {
   int code_tab = -1;              // code compiled into the code-specialized int8 kernel, -1 if none
   bool beta_tab_code_ok = false;  // the offset table suits that kernel (every entry 1..127)
   int8_t* d_beta_tab8 = nullptr;  // the same table as int8 for the row-pair kernel, when it suits it
-                                  // (entries 1..127, fixed-Z shape, at most 4096 entries)
+                                  // (entries 0..127, fixed-Z shape, at most 4096 entries)
 };
 
 int pair_i8_shape(const DecoderDev& d);

@@ -86,7 +86,7 @@ struct PairArgs {
   long long chunk;         // codewords per wave (persistent grid)
   int max_blocks;          // resident-grid size bound (0: one block per cw_per_wave*waves codewords)
   int beta, cap;           // int8 beta (1..127) and cap (0..127)
-  const int8_t* beta_tab;  // [max_iter][mrow] per-(iteration, layer) int8 offsets (1..127) or null
+  const int8_t* beta_tab;  // [max_iter][mrow] per-(iteration, layer) int8 offsets (0..127) or null
   const int8_t* llr_in;    // dense [num_cw][mcol*z]
   int8_t* post_out;
   uint32_t* iters;
@@ -225,11 +225,15 @@ k_pair_i8(const PairArgs a) {
       constexpr int MAXW = SH::maxw;
       const int w = SH::w[l];
       ushort2v beta2l = beta2, capb2l = capb2;
+      // |Q| mask: with an offset of 0 the reference's abs(-128) = -128 gives A = max(-128, 0)
+      // = 0 (decode_functions.cpp:56-61), and |Q| = 0x8000 is the only magnitude with bit 15
+      uint32_t amask = 0xffffffffu;
       if constexpr (BT) {   // the layer's offset b: beta2 = b << 8, capb2 = min(cap + b, 128) << 8
         const uint32_t b = bt_it[l];
         const uint32_t cb = b + (uint32_t)a.cap;
         beta2l = as_us(b * 0x01000100u);
         capb2l = as_us((cb < 128u ? cb : 128u) * 0x01000100u);
+        amask = b ? 0xffffffffu : 0x7fff7fffu;
       }
       // Min1/min2 are tracked on |Q_j| (unsigned, |-128| = 128).  With beta >= 1,
       // A_j = max(|Q_j| - beta, 0) and the capped exclusive minimum is, per row,
@@ -252,7 +256,7 @@ k_pair_i8(const PairArgs a) {
           const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(c2v_get(c2vp, e0 + j)));
           const short2v qq = as_s(as_u(qs) & 0xff00ff00u);   // Q << 8, Q = sat8(L - R)
           q[j] = qq;
-          const ushort2v av = as_us(as_u(smax(qq, -qq)));
+          const ushort2v av = BT ? as_us(as_u(smax(qq, -qq)) & amask) : as_us(as_u(smax(qq, -qq)));
           A[j] = av;
           if (j & 1) S = xor3(S, as_u(q[j - 1]), as_u(qq));
           else if (j + 1 == w) S ^= as_u(qq);

@@ -468,8 +468,8 @@ def test_gpu_global_c2v_codes_match_oracle(case, cuda):
 def test_gpu_pair_kernel_offset_tables(case, cuda):
     """Per-(iteration, layer) offset tables on the row-pair kernel (802.11ad rate 1/2,
     entries 1..4): posteriors, C2V and iteration counts equal the oracle's with the same
-    table; the same decoder with a constant table equals the plain decoder; a table with
-    a 0 entry or a lifted Z falls back to the generic kernel."""
+    table; the same decoder with a constant table equals the plain decoder; entries of 0
+    stay on the pair kernel, a negative entry falls back to the generic kernel."""
     import torch
     from fecsim import lib as L
     snr, mi, cap, ncw = case
@@ -504,15 +504,20 @@ def test_gpu_pair_kernel_offset_tables(case, cuda):
     d.decode(x, out, iters=it)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), plain[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), plain[2])
-    # an entry of 0 (the abs(-128) wrap path): generic kernel, same results as the oracle
-    tab0 = tab.copy()
-    tab0[0, 3] = 0.0
-    d.set_offsets(tab0)
-    assert d.kernel == L.KERNEL_GENERIC
-    want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab0)
-    d.decode(x, out, iters=it)
-    torch.cuda.synchronize()
-    assert np.array_equal(out.cpu().numpy(), want0[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), want0[2])
+    # entries of 0 (the abs(-128) wrap path: a saturated -128 gets magnitude 0) on the pair
+    # kernel; a negative entry falls back to the generic kernel; both equal the oracle
+    for bad, kern in ((0.0, L.KERNEL_PAIR), (-1.0, L.KERNEL_GENERIC)):
+        tab0 = tab.copy()
+        tab0[0, 3] = bad
+        tab0[mi - 1, :] = bad
+        d.set_offsets(tab0)
+        assert d.kernel == kern
+        want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab0)
+        d.decode(x, out, iters=it, c2v=c2v)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want0[0])
+        assert np.array_equal(it.cpu().numpy().astype(np.uint32), want0[2])
+        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want0[1])
 
 
 def test_gpu_pair_kernel_table_lifted_z_falls_back(cuda):
