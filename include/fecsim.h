@@ -407,16 +407,23 @@ typedef struct {
     uint32_t iters;      /* T */
     uint32_t max_batch;  /* codewords per call at most (sizes the checkpoints) */
     float cap;           /* row cap (max_row_metric) */
-    int elem;            /* FECSIM_F32 or FECSIM_F64 */
+    int elem;            /* FECSIM_F32 or FECSIM_F64; or FECSIM_I8: float32 tensors emulating the
+                            int8 decoder (Vec64c_min) -- LLRs given as integer-valued floats in
+                            the decoder's units, V2C values and posteriors saturated to
+                            [-128, 127], offsets rounded to integers (the gradient passes the
+                            rounding unchanged, a saturated value passes none), an offset of 0
+                            giving A = 0 at -128 as abs(-128) does; alpha should be 1 */
 } fecsim_train_params;
 
 int fecsim_trainer_create(const fecsim_hdec *h, const fecsim_train_params *p, fecsim_trainer **out);
-/* llr, post: device [batch][mcol*z]; alpha, beta: device [T][mrow]; all of elem's type */
+/* llr, post: device [batch][mcol*z]; alpha, beta: device [T][mrow]; all of elem's type
+ * (float32 for FECSIM_I8).  The trainer copies alpha and beta on `stream` for the
+ * backward: the caller's buffers may be freed or changed once the call is ordered. */
 int fecsim_train_fwd(fecsim_trainer *t, uint32_t batch, const void *llr, const void *alpha, const void *beta,
                      void *post, void *stream);
-/* Gradients of a loss L given dL/d(post) of the last forward (same batch, llr, alpha,
- * beta): grad_alpha, grad_beta [T][mrow] (overwritten, summed over the batch),
- * grad_llr [batch][mcol*z] or NULL. */
+/* Gradients of a loss L given dL/d(post) of the last forward (same batch; one backward
+ * per forward, FECSIM_E_INVAL otherwise): grad_alpha, grad_beta [T][mrow] (overwritten,
+ * summed over the batch), grad_llr [batch][mcol*z] or NULL. */
 int fecsim_train_bwd(fecsim_trainer *t, uint32_t batch, const void *grad_post, void *grad_alpha, void *grad_beta,
                      void *grad_llr, void *stream);
 void fecsim_trainer_free(fecsim_trainer *t);

@@ -150,6 +150,7 @@ struct TrainArgs {
   int maxw;                  // largest row weight
   int waves;                 // codewords (waves) per block
   int is_double, global_p;   // element type; posteriors in global scratch instead of LDS
+  int quant;                 // float32 emulating the int8 decoder (saturation, integer offsets)
   double cap;
   uint64_t batch;
   const void* llr;           // [batch][n]

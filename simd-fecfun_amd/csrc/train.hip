@@ -28,6 +28,8 @@
 // first copies the code's edge words and layer weights into the front of its LDS.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "decode_core.h"
 #include "internal.h"
 
@@ -60,9 +62,24 @@ __device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t 
   return v;
 }
 
-// one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q.
-// MAXW bounds the row weight w (the unrolled loops stop there).
-template <typename F, int MAXW>
+// int8 decoder emulation (QT): saturation of V2C and posteriors to int8, and the
+// magnitude of an offset-0 row: abs(-128) = -128 gives A = max(-128, 0) = 0
+// (decode_functions.cpp:48-61,88-90).  On integer-valued inputs with integer offsets
+// every value stays an integer, exact in float.
+template <typename F>
+__device__ __forceinline__ F sat8f(F x) { return x < F(-128) ? F(-128) : (x > F(127) ? F(127) : x); }
+template <typename F, bool QT>
+__device__ __forceinline__ F mag_a(F q, F be) {
+  const F x = fabs(q) - be;
+  const F A = x > F(0) ? x : F(0);
+  if constexpr (QT) return (be == F(0) && q == F(-128)) ? F(0) : A;
+  return A;
+}
+
+// one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q
+// (with QT: the V2C value before its saturation). MAXW bounds the row weight w (the
+// unrolled loops stop there).
+template <typename F, int MAXW, bool QT>
 __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __restrict__ edges, F* P, const F* Cold,
                                         F* Cnew, F* Qs, int e0, int w, int r, F al, F be, F cap) {
   const int Z = a.z;
@@ -75,10 +92,10 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
     if (j < w) {
       vi[j] = edge_var(edges[e0 + j], r, Z);
       const int ce = (e0 + j) * Z + r;
-      q[j] = P[vi[j]] - Cold[ce];
-      if (Qs) Qs[ce] = q[j];
-      const F x = fabs(q[j]) - be;
-      const F A = x > F(0) ? x : F(0);
+      const F xq = P[vi[j]] - Cold[ce];
+      if (Qs) Qs[ce] = xq;
+      q[j] = QT ? sat8f(xq) : xq;
+      const F A = mag_a<F, QT>(q[j], be);
       sg ^= sbit(q[j]);
       if (A < m1) { m2 = m1; m1 = A; im = j; }
       else if (A < m2) m2 = A;
@@ -91,7 +108,7 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
       const F m = al * (cap < x ? cap : x);
       const bool neg = sg ^ sbit(q[j]);
       const F rn = neg ? -m : m;
-      P[vi[j]] = q[j] + rn;
+      P[vi[j]] = QT ? sat8f(q[j] + rn) : q[j] + rn;
       if (Cnew) Cnew[(e0 + j) * Z + r] = rn;
     }
   }
@@ -99,20 +116,24 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
 
 // one check row of the reverse sweep (see the head of this file); adds the row's
 // parameter gradients to *ga, *gb
-template <typename F, int MAXW>
+// with QT a saturated value passes no gradient (the clamps), and the offset's rounding
+// passes it straight through (d round(b) / db taken as 1)
+template <typename F, int MAXW, bool QT>
 __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* gP, F* gC, const F* Qs, int e0, int w,
                                         int r, int Z, F al, F be, F cap, F* ga, F* gb) {
   F q[MAXW], gq[MAXW];
   int vi[MAXW];
+  bool qin[MAXW];   // QT: the V2C value was inside the int8 range (its clamp passes gradient)
   F m1 = F(INFINITY), m2 = F(INFINITY);
   int im = -1, im2 = -1, sgn = 0;
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
       vi[j] = edge_var(edges[e0 + j], r, Z);
-      q[j] = Qs[(e0 + j) * Z + r];
-      const F x = fabs(q[j]) - be;
-      const F A = x > F(0) ? x : F(0);
+      const F xq = Qs[(e0 + j) * Z + r];
+      q[j] = QT ? sat8f(xq) : xq;
+      qin[j] = !QT || (xq >= F(-128) && xq <= F(127));
+      const F A = mag_a<F, QT>(q[j], be);
       sgn ^= sbit(q[j]);
       if (A < m1) { m2 = m1; im2 = im; m1 = A; im = j; }
       else if (A < m2) { m2 = A; im2 = j; }
@@ -129,7 +150,11 @@ __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* g
       const bool capped = cap < x;
       const F mc = capped ? cap : x;
       const F s = (sgn ^ sbit(q[j])) ? F(-1) : F(1);
-      const F gv = gP[vi[j]];
+      F gv = gP[vi[j]];
+      if constexpr (QT) {   // the posterior's saturation
+        const F y = q[j] + s * (al * mc);
+        if (y < F(-128) || y > F(127)) gv = F(0);
+      }
       const F gm = s * (gv + gC[ce]);
       gq[j] = gv;
       gaa += gm * mc;
@@ -143,12 +168,14 @@ __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* g
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
       const F gA = (j == im) ? g_im : ((j == im2) ? g_im2 : F(0));
-      if (fabs(q[j]) - be > F(0)) {
+      const bool wrap = QT && be == F(0) && q[j] == F(-128);   // A = 0 there, constant
+      if (fabs(q[j]) - be > F(0) && !wrap) {
         gq[j] += sbit(q[j]) ? -gA : gA;
         gbb -= gA;
       }
-      gP[vi[j]] = gq[j];
-      gC[(e0 + j) * Z + r] = -gq[j];
+      const F g = qin[j] ? gq[j] : F(0);
+      gP[vi[j]] = g;
+      gC[(e0 + j) * Z + r] = -g;
     }
   }
   *ga = gaa;
@@ -162,7 +189,14 @@ __device__ __forceinline__ F wave_sum(F v) {
   return v;
 }
 
-template <typename F, bool GP, int MAXW>
+// QT offsets: rounded to integers (the int8 decoder's table takes integers)
+template <typename F, bool QT>
+__device__ __forceinline__ F offset_of(F b) {
+  if constexpr (QT) return rint(b);
+  return b;
+}
+
+template <typename F, bool GP, int MAXW, bool QT>
 __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -200,8 +234,8 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
     int e0 = 0;
     for (int l = 0; l < a.mrow; ++l) {
       const int w = layer_w[l];
-      const F al = alpha[t * a.mrow + l], be = beta[t * a.mrow + l];
-      for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
+      const F al = alpha[t * a.mrow + l], be = offset_of<F, QT>(beta[t * a.mrow + l]);
+      for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW, QT>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
       e0 += w;
       wave_sync();
     }
@@ -210,7 +244,7 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   for (int i = lane; i < N; i += 64) out[i] = P[i];
 }
 
-template <typename F, bool GP, int MAXW>
+template <typename F, bool GP, int MAXW, bool QT>
 __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -254,8 +288,8 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
         int e0 = 0;
         for (int l = 0; l < Lr; ++l) {
           const int w = layer_w[l];
-          const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
-          for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
+          const F al = alpha[t * Lr + l], be = offset_of<F, QT>(beta[t * Lr + l]);
+          for (int r = lane; r < Z; r += 64) fwd_row<F, MAXW, QT>(a, edges, P, C, C, Qs, e0, w, r, al, be, cap);
           e0 += w;
           wave_sync();
         }
@@ -267,9 +301,9 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
       for (int l = Lr - 1; l >= 0; --l) {
         const int w = layer_w[l];
         e0 -= w;
-        const F al = alpha[t * Lr + l], be = beta[t * Lr + l];
+        const F al = alpha[t * Lr + l], be = offset_of<F, QT>(beta[t * Lr + l]);
         F ga = F(0), gb = F(0);
-        for (int r = lane; r < Z; r += 64) rev_row<F, MAXW>(edges, gP, gC, Qs, e0, w, r, Z, al, be, cap, &ga, &gb);
+        for (int r = lane; r < Z; r += 64) rev_row<F, MAXW, QT>(edges, gP, gC, Qs, e0, w, r, Z, al, be, cap, &ga, &gb);
         wave_sync();
         ga = wave_sum(ga);
         gb = wave_sum(gb);
@@ -308,8 +342,15 @@ template <typename F, bool GP, int MAXW>
 hipError_t launch_tw(const TrainArgs& a, int bwd, hipStream_t st) {
   const uint64_t blocks = (a.batch + a.waves - 1) / a.waves;
   const size_t lds = (size_t)a.tab_bytes + (GP ? 0 : (size_t)a.waves * 2 * a.n_pad * sizeof(F));
-  if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP, MAXW>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
-  else hipLaunchKernelGGL((k_train_fwd<F, GP, MAXW>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  if constexpr (std::is_same<F, float>::value) {
+    if (a.quant) {
+      if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP, MAXW, true>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+      else hipLaunchKernelGGL((k_train_fwd<F, GP, MAXW, true>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+      return hipGetLastError();
+    }
+  }
+  if (bwd) hipLaunchKernelGGL((k_train_bwd<F, GP, MAXW, false>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
+  else hipLaunchKernelGGL((k_train_fwd<F, GP, MAXW, false>), dim3((uint32_t)blocks), dim3(64 * a.waves), lds, st, a);
   return hipGetLastError();
 }
 

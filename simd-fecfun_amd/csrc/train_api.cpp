@@ -24,10 +24,12 @@ struct fecsim_trainer {
   void* gc = nullptr;
   void* partial = nullptr;
   void* gscratch = nullptr;
-  // arguments of the last forward, reused by the adjoint
+  // the last forward's batch and its alpha, beta (copied on the forward's stream, so the
+  // adjoint does not depend on the caller keeping its buffers)
   uint32_t last_batch = 0;
-  const void* alpha = nullptr;
-  const void* beta = nullptr;
+  int have_fwd = 0;
+  void* alpha = nullptr;
+  void* beta = nullptr;
 };
 
 namespace {
@@ -44,6 +46,8 @@ void free_all(fecsim_trainer* t) {
   (void)hipFree(t->gc);
   (void)hipFree(t->partial);
   (void)hipFree(t->gscratch);
+  (void)hipFree(t->alpha);
+  (void)hipFree(t->beta);
 }
 
 TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
@@ -62,6 +66,7 @@ TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
   a.maxw = h.max_row_weight;
   a.waves = t->waves;
   a.is_double = t->p.elem == FECSIM_F64;
+  a.quant = t->p.elem == FECSIM_I8;
   a.global_p = t->global_p;
   a.cap = (double)t->p.cap;
   a.batch = batch;
@@ -82,7 +87,8 @@ extern "C" {
 
 int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, fecsim_trainer** out) {
   if (!hd || !p || !out) return fail(FECSIM_E_INVAL, "null argument");
-  if (p->elem != FECSIM_F32 && p->elem != FECSIM_F64) return fail(FECSIM_E_INVAL, "elem must be FECSIM_F32 or FECSIM_F64");
+  if (p->elem != FECSIM_F32 && p->elem != FECSIM_F64 && p->elem != FECSIM_I8)
+    return fail(FECSIM_E_INVAL, "elem must be FECSIM_F32, FECSIM_F64 or FECSIM_I8");
   if (p->iters == 0 || p->iters > 1000 || p->max_batch == 0) return fail(FECSIM_E_INVAL, "need 1..1000 iterations and max_batch >= 1");
   const HdecHost& h = hd->h;
   if (h.max_row_weight > kTrainMaxW) return fail(FECSIM_E_UNSUPPORTED, "row weight above 32");
@@ -123,6 +129,8 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (he == hipSuccess) he = hipMalloc(&t->gc, B * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->partial, B * T * h.mrow * 2 * eb);
   if (he == hipSuccess && t->global_p) he = hipMalloc(&t->gscratch, B * 2 * N * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->alpha, T * h.mrow * eb);
+  if (he == hipSuccess) he = hipMalloc(&t->beta, T * h.mrow * eb);
   if (he != hipSuccess) {
     free_all(t);
     delete t;
@@ -137,22 +145,25 @@ int fecsim_train_fwd(fecsim_trainer* t, uint32_t batch, const void* llr, const v
   if (!t || ((!llr || !alpha || !beta || !post) && batch)) return fail(FECSIM_E_INVAL, "null argument");
   if (batch > t->p.max_batch) return fail(FECSIM_E_INVAL, "batch above max_batch");
   TrainArgs a = base_args(t, batch);
+  const size_t pbytes = (size_t)t->p.iters * t->h.mrow * t->elem_bytes;
+  hipError_t e = hipMemcpyAsync(t->alpha, alpha, pbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(t->beta, beta, pbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "train forward parameter copy");
   a.llr = llr;
-  a.alpha = alpha;
-  a.beta = beta;
+  a.alpha = t->alpha;
+  a.beta = t->beta;
   a.post = post;
-  const hipError_t e = launch_train(a, 0, (hipStream_t)stream);
+  e = launch_train(a, 0, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "train forward launch");
   t->last_batch = batch;
-  t->alpha = alpha;
-  t->beta = beta;
+  t->have_fwd = 1;
   return FECSIM_OK;
 }
 
 int fecsim_train_bwd(fecsim_trainer* t, uint32_t batch, const void* grad_post, void* grad_alpha, void* grad_beta,
                      void* grad_llr, void* stream) {
   if (!t || !grad_alpha || !grad_beta || (!grad_post && batch)) return fail(FECSIM_E_INVAL, "null argument");
-  if (batch != t->last_batch || !t->alpha) return fail(FECSIM_E_INVAL, "backward needs the forward of the same batch first");
+  if (batch != t->last_batch || !t->have_fwd) return fail(FECSIM_E_INVAL, "backward needs a forward of the same batch since the last backward");
   TrainArgs a = base_args(t, batch);
   a.alpha = t->alpha;
   a.beta = t->beta;
@@ -162,6 +173,7 @@ int fecsim_train_bwd(fecsim_trainer* t, uint32_t batch, const void* grad_post, v
   a.grad_llr = grad_llr;
   const hipError_t e = launch_train(a, 1, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "train backward launch");
+  t->have_fwd = 0;   // the recomputation overwrote the kept V2C values of the last segment
   return FECSIM_OK;
 }
 

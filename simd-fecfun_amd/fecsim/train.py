@@ -27,11 +27,15 @@ def _stream():
 
 
 class Trainer:
-    """Device state of the unrolled decoder: checkpoints for up to max_batch codewords."""
+    """Device state of the unrolled decoder: checkpoints for up to max_batch codewords.
+    int8=True: float32 emulation of the int8 decoder (integer-valued LLRs in the
+    decoder's units, saturation to [-128, 127], offsets rounded with a straight-through
+    gradient)."""
 
-    def __init__(self, hdec: Hdec, iters: int, max_batch: int, cap: float = 1e30, double: bool = False):
+    def __init__(self, hdec: Hdec, iters: int, max_batch: int, cap: float = 1e30, double: bool = False,
+                 int8: bool = False):
         self.hdec, self.iters, self.max_batch, self.cap = hdec, iters, max_batch, cap
-        self.elem = _L.F64 if double else _L.F32
+        self.elem = _L.I8 if int8 else (_L.F64 if double else _L.F32)
         p = TrainParams(iters, max_batch, cap, self.elem)
         t = C.c_void_p()
         _L.check(_L.lib().fecsim_trainer_create(hdec._h, C.byref(p), C.byref(t)), "fecsim_trainer_create")
@@ -146,8 +150,90 @@ def train(cs, henc, T=10, batch=4096, steps=400, ebn0=(1.0, 3.0), lr=1e-3, init_
         losses.append(float(loss.item()))
         if log and (step % log_every == 0 or step == steps - 1):
             log("step %d loss %.5f beta[0] %.3f..%.3f alpha[0] %.3f" % (
-                step, losses[-1], float(beta[0].min()), float(beta[0].max()), float(alpha[0].mean())))
+                step, losses[-1], float(beta.detach()[0].min()), float(beta.detach()[0].max()),
+                float(alpha.detach()[0].mean())))
     return alpha.detach().cpu().numpy(), beta.detach().cpu().numpy(), losses
+
+
+def _int8_batch(henc, batch, ebn0, rate, seed, cw0, rng, llr_scale, max_abs):
+    """Random codewords and the int8 decoder's LLRs (as float32 integers); the batch is
+    split into 8 slices, each at an Eb/N0 drawn uniformly from ebn0.  Returns (llr, bits01)."""
+    import torch
+    bits = henc.new_bits(batch)
+    henc.gen_codewords(bits, seed=seed, cw0=cw0)
+    q = torch.empty((batch, henc.n), dtype=torch.int8, device="cuda")
+    edges = np.linspace(0, batch, 9).astype(int)
+    for g in range(8):
+        a, b = edges[g], edges[g + 1]
+        if b <= a:
+            continue
+        snr = float(rng.uniform(ebn0[0], ebn0[1]) + rate)
+        henc.gen_llr(bits[a:b], q[a:b], _L.I8, snr, seed=seed, cw0=cw0 + a, llr_scale=llr_scale, max_abs_llr=max_abs)
+    z, mcol = henc.z, henc.mcol
+    shifts = torch.arange(32, device="cuda", dtype=torch.int32)
+    b01 = ((bits.unsqueeze(-1) >> shifts) & 1).reshape(batch, mcol, -1)[:, :, :z].reshape(batch, mcol * z)
+    return q.to(torch.float32), b01.to(torch.float32)
+
+
+def train_int8(cs, henc, T=10, batch=512, steps=400, ebn0=(0.5, 1.0), lr=2e-2, init=None, seed=7, log_every=50,
+               log=None, llr_scale=2.0, max_abs=24.0, cap=31.0, rate=None, greedy=False, loss_iters=None):
+    """Offsets for the int8 decoder (Vec64c_min, LLR scale 2, |LLR| <= 24, cap 31) trained
+    through the int8-emulating forward: the LLRs are the int8 decoder's own (integers),
+    the decoder saturates as the int8 one does and rounds the offsets (straight-through
+    gradient), so the table that training sees is the table the bit-exact decoder runs.
+    Adam on the BCE of the posteriors (in LLR units, divided by llr_scale) against the
+    transmitted bits, offsets clamped to [0, 20].
+
+    greedy=False: all T iterations' offsets trained together on the loss after T
+    iterations.  greedy=True: iteration by iteration -- for t = 1..T, row t-1 of the
+    table (started from row t-2) is trained for steps // T steps on the loss after t
+    iterations, the earlier rows held fixed; every iteration's offsets then see a loss
+    whose posteriors are not yet saturated.  loss_iters (joint training only): the loss is
+    the mean of the BCE after each of these iteration counts (default [T]), so early
+    iterations' offsets also get gradient from posteriors that are not yet saturated.
+    Returns (integer offsets [T, mrow], losses)."""
+    import torch
+    import torch.nn.functional as Fn
+    hdec = Hdec.from_tables(cs.dec)
+    Lr = cs.dec.mrow
+    tab = np.ones((T, Lr), np.float32) if init is None else np.array(init, np.float32)
+    fn = min_sum_function()
+    rng = np.random.default_rng(seed)
+    if rate is None:   # Es/N0 = Eb/N0 + rate (dB); pass the transmitted rate for punctured codes
+        rate = rate_db(cs.dec.mcol * cs.z - cs.enc.mrow * cs.z, cs.dec.mcol * cs.z)
+    losses = []
+    stages = [(t, t - 1, max(1, steps // T)) for t in range(1, T + 1)] if greedy else [(T, None, steps)]
+    step = 0
+    for t, row, n in stages:
+        ks = [t] if greedy or not loss_iters else sorted(set(int(k) for k in loss_iters) | {T})
+        if ks[0] < 1 or ks[-1] > T:
+            raise ValueError("loss_iters must lie in 1..T")
+        trs = [(k, Trainer(hdec, k, batch, cap=cap, int8=True)) for k in ks]
+        if row is not None and row > 0 and init is None:
+            tab[row] = tab[row - 1]
+        lo = 0 if row is None else row
+        fixed = torch.tensor(tab[:lo], device="cuda")
+        beta = torch.tensor(tab[lo:t], device="cuda", requires_grad=True)
+        alpha = torch.ones((t, Lr), device="cuda")
+        opt = torch.optim.Adam([beta], lr=lr)
+        for _ in range(n):
+            q, b01 = _int8_batch(henc, batch, ebn0, rate, seed, step * batch, rng, llr_scale, max_abs)
+            full = torch.cat([fixed, beta])
+            loss = sum(Fn.binary_cross_entropy_with_logits(-fn.apply(q, alpha[:k], full[:k], tr) / llr_scale, b01)
+                       for k, tr in trs) / len(trs)
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                beta.clamp_(min=0.0, max=20.0)
+            losses.append(float(loss.item()))
+            if log and (step % log_every == 0 or step == steps - 1):
+                r = np.rint(beta.detach().cpu().numpy()).astype(int)
+                log("step %d iters %d loss %.5f offsets %s" % (step, t, losses[-1], np.bincount(r.ravel())))
+            step += 1
+        tab[lo:t] = beta.detach().cpu().numpy()
+        del trs
+    return np.rint(tab).astype(np.float32), losses
 
 
 def quantize_offsets(beta, llr_scale=2.0, max_iter=None):
@@ -375,13 +461,39 @@ def _main():
     ap.add_argument("--search-from", default="trained",
                     help="start of the search: 'trained' (the quantized trained table), 'plain' (beta = 1 "
                          "everywhere) or the path of an earlier --out file (its integer table)")
+    ap.add_argument("--int8-aware", action="store_true",
+                    help="train the int8 decoder's integer offsets through its float emulation (implies --offset-only)")
+    ap.add_argument("--greedy", action="store_true", help="int8-aware: train iteration by iteration")
+    ap.add_argument("--loss-iters", default=None, help="int8-aware: iteration counts whose losses are averaged")
+    ap.add_argument("--init-from", default=None,
+                    help="int8-aware: start table, a committed table's name (fecsim/learned/<name>.json) or the "
+                         "path of an earlier --out file (default: beta = 1 everywhere)")
+    ap.add_argument("--rate-db", type=float, default=None,
+                    help="int8-aware: 10 log10(K/N) of the transmitted code (Es/N0 = Eb/N0 + rate-db)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
+    if args.int8_aware:
+        args.offset_only = True
     log = lambda s: print(s, file=sys.stderr, flush=True)   # noqa: E731
     cs = codes.gen_structs(args.code, simd_size=512, elem_size=8)
     henc = Henc.from_tables(cs.enc)
     t0 = time.time()
-    if args.steps > 0:
+    if args.steps > 0 and args.int8_aware:
+        li = [int(x) for x in args.loss_iters.split(",")] if args.loss_iters else None
+        init = None
+        if args.init_from:
+            if args.init_from.endswith(".json"):
+                with open(args.init_from) as f:
+                    init = np.asarray(json.load(f)["beta_int"], np.float32)
+            else:
+                init = load_offsets(args.init_from)[0]
+            init = init[:args.T]
+        tab, losses = train_int8(cs, henc, T=args.T, batch=args.batch, steps=args.steps, ebn0=tuple(args.ebn0),
+                                 lr=args.lr, init=init, seed=args.seed, log=log, log_every=max(1, args.steps // 20),
+                                 rate=args.rate_db, greedy=args.greedy, loss_iters=li)
+        alpha = np.ones_like(tab)
+        beta = tab / 2.0   # LLR units of scale 1: quantize_offsets gives back the trained integers
+    elif args.steps > 0:
         alpha, beta, losses = train(cs, henc, T=args.T, batch=args.batch, steps=args.steps, ebn0=tuple(args.ebn0),
                                     lr=args.lr, init_offset=args.init_offset, init_scale=args.init_scale,
                                     learn_scale=not args.offset_only, cap=args.cap, seed=args.seed, log=log,
@@ -392,7 +504,8 @@ def _main():
         losses = []
     res = {"code": args.code, "T": args.T, "batch": args.batch, "steps": args.steps, "ebn0": list(args.ebn0),
            "lr": args.lr, "init_offset": args.init_offset, "init_scale": args.init_scale,
-           "offset_only": args.offset_only, "cap": args.cap, "seed": args.seed, "train_seconds": time.time() - t0,
+           "offset_only": args.offset_only, "int8_aware": args.int8_aware, "greedy": args.greedy,
+           "loss_iters": args.loss_iters, "init_from": args.init_from, "rate_db": args.rate_db, "cap": args.cap, "seed": args.seed, "train_seconds": time.time() - t0,
            "loss_first": losses[:5], "loss_last": losses[-5:], "alpha": alpha.tolist(), "beta": beta.tolist()}
     if args.eval_snr:
         lo, hi, st = args.eval_snr

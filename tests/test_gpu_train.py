@@ -220,3 +220,117 @@ def test_gpu_offset_search(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
     assert np.array_equal(out.cpu().numpy(), want[0])
+
+
+@pytest.mark.parametrize("name,T,snr", [("11ad 1/2", 6, -1.5), ("11n 1/2 z81", 8, -0.5)])
+def test_gpu_train_int8_forward_matches_int8_decoder(name, T, snr, cuda):
+    """The int8-emulating forward (FECSIM_I8 trainer) on int8 LLRs with an integer offset
+    table equals the bit-exact int8 decoder (the oracle, same table) on every codeword
+    that decoder ran for all T iterations."""
+    import torch
+    from fecsim.train import Trainer
+    cs = _cs(name)
+    llr = _llrs(cs, 300, snr, O.I8, 2.0, 24.0)
+    rng = np.random.default_rng(12)
+    tab = rng.integers(0, 4, (T, cs.dec.mrow)).astype(np.float32)
+    want = O.decode_ideal(cs.dec, llr, max_iter=T, max_row_metric=31.0, beta=1.0, beta_table=tab)
+    full = want[2] == T
+    assert full.sum() >= 20, int(full.sum())
+    tr = Trainer(Hdec.from_tables(cs.dec), T, 300, cap=31.0, int8=True)
+    x = torch.from_numpy(llr.astype(np.float32)).cuda()
+    al = torch.ones((T, cs.dec.mrow), device="cuda")
+    got = tr.forward(x, al, torch.from_numpy(tab + 0.3).cuda())   # rounds to the table
+    torch.cuda.synchronize()
+    g = got.cpu().numpy()
+    assert np.array_equal(g, np.rint(g))
+    assert np.array_equal(g[full].astype(np.int8), want[0][full])
+
+
+def test_gpu_train_int8_gradient_matches_torch_reference(cuda):
+    """Gradients of the int8-emulating decoder (saturation, rounded offsets with a
+    straight-through gradient, the offset-0 wrap) against autograd of the independent
+    torch restatement in the same mode."""
+    import torch
+    from fecsim.train import Trainer, min_sum_function
+    cs = _cs("11ad 1/2")
+    T = 5
+    x = _llrs(cs, 6, 1.0, O.I8, 2.0, 24.0).astype(np.float32) * 4   # large LLRs: saturation binds
+    x = np.clip(x, -128, 127)
+    rng = np.random.default_rng(21)
+    be = rng.integers(0, 4, (T, cs.dec.mrow)).astype(np.float64) + rng.uniform(-0.3, 0.3, (T, cs.dec.mrow))
+    be = np.clip(be, 0.0, None)
+    bits = (rng.random(x.shape) < 0.5).astype(np.float64)
+    tr = Trainer(Hdec.from_tables(cs.dec), T, 8, cap=31.0, int8=True)
+    fn = min_sum_function()
+    tb = torch.tensor(be, dtype=torch.float32, device="cuda", requires_grad=True)
+    tx = torch.tensor(x, device="cuda", requires_grad=True)
+    ta = torch.ones((T, cs.dec.mrow), device="cuda")
+    post = fn.apply(tx, ta, tb, tr)
+    loss = torch.nn.functional.binary_cross_entropy_with_logits(-post / 2, torch.tensor(bits, dtype=torch.float32, device="cuda"))
+    loss.backward()
+    rb = torch.tensor(be, requires_grad=True)
+    rx = torch.tensor(x.astype(np.float64), requires_grad=True)
+    rp = train_ref.forward(cs.dec, rx, torch.ones((T, cs.dec.mrow), dtype=torch.float64), rb, cap=31.0, int8=True)
+    assert np.array_equal(post.detach().cpu().numpy(), rp.detach().numpy().astype(np.float32))
+    assert (np.abs(rp.detach().numpy()) == 127).any() or (rp.detach().numpy() == -128).any()   # saturation occurred
+    rl = torch.nn.functional.binary_cross_entropy_with_logits(-rp / 2, torch.tensor(bits))
+    rl.backward()
+    np.testing.assert_allclose(tb.grad.cpu().numpy(), rb.grad.numpy(), rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(tx.grad.cpu().numpy(), rx.grad.numpy(), rtol=1e-4, atol=1e-7)
+
+
+@pytest.mark.parametrize("greedy,loss_iters", [(True, None), (False, [1, 2])])
+def test_gpu_train_int8_schedules(greedy, loss_iters, cuda):
+    """train_int8 (greedy, and joint with a multi-iteration loss) on 11ad 1/2 at T = 3:
+    one finite loss per step, an integer table of shape [T, mrow] within [0, 20], and
+    the bit-exact decoder with that table decodes like the oracle with it."""
+    import torch
+    from fecsim.sim import Henc
+    from fecsim.train import train_int8
+    cs = _cs("11ad 1/2")
+    henc = Henc.from_tables(cs.enc)
+    T = 3
+    tab, losses = train_int8(cs, henc, T=T, batch=64, steps=6, ebn0=(0.0, 1.0), lr=0.5, greedy=greedy,
+                             loss_iters=loss_iters)
+    assert len(losses) == 6 and np.isfinite(losses).all()
+    assert tab.shape == (T, cs.dec.mrow) and np.array_equal(tab, np.rint(tab))
+    assert tab.min() >= 0 and tab.max() <= 20
+    with pytest.raises(ValueError):
+        train_int8(cs, henc, T=T, batch=64, steps=1, loss_iters=[T + 1])
+    llr = _llrs(cs, 200, -0.5, O.I8, 2.0, 24.0)
+    want = O.decode_ideal(cs.dec, llr, max_iter=T, max_row_metric=31.0, beta=1.0, beta_table=tab)
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", T, 31.0, 1.0)
+    d.set_offsets(tab)
+    x = torch.from_numpy(llr).cuda()
+    out = torch.empty_like(x)
+    d.decode(x, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), want[0])
+
+
+def test_gpu_train_backward_keeps_its_parameters(cuda):
+    """The backward uses the forward's alpha and beta even when the caller's tensors were
+    temporaries freed (and their memory reused) in between; a second backward without a
+    new forward is refused."""
+    import torch
+    from fecsim.lib import FecsimError
+    from fecsim.train import Trainer
+    cs = _cs("11ad 1/2")
+    T = 5
+    x = torch.tensor(_llrs(cs, 16, 0.0).astype(np.float32), device="cuda")
+    rng = np.random.default_rng(5)
+    al = torch.tensor(rng.uniform(0.6, 1.0, (T, cs.dec.mrow)), dtype=torch.float32, device="cuda")
+    be = torch.tensor(rng.uniform(0.1, 0.9, (T, cs.dec.mrow)), dtype=torch.float32, device="cuda")
+    w = torch.tensor(rng.normal(size=tuple(x.shape)), dtype=torch.float32, device="cuda")
+    tr = Trainer(Hdec.from_tables(cs.dec), T, 16, cap=8.0)
+    tr.forward(x, al, be)
+    _, ga, gb = tr.backward(w)
+    tr.forward(x, al.clone(), be.clone())   # temporaries, freed when the call returns
+    junk = [torch.full_like(al, 1e9) for _ in range(64)]   # takes their memory back
+    _, ga2, gb2 = tr.backward(w)
+    torch.cuda.synchronize()
+    del junk
+    assert np.array_equal(ga.cpu().numpy(), ga2.cpu().numpy())
+    assert np.array_equal(gb.cpu().numpy(), gb2.cpu().numpy())
+    with pytest.raises(FecsimError):
+        tr.backward(w)

@@ -23,8 +23,13 @@ def layer_index(dec):
     return out
 
 
-def forward(dec, llr, alpha, beta, cap=1e30, idx=None):
-    """llr [B, N], alpha/beta [T, mrow] tensors (any may require grad) -> posteriors [B, N]."""
+def forward(dec, llr, alpha, beta, cap=1e30, idx=None, int8=False):
+    """llr [B, N], alpha/beta [T, mrow] tensors (any may require grad) -> posteriors [B, N].
+
+    int8: the int8 decoder's arithmetic on integer-valued inputs -- V2C values and
+    posteriors clamped to [-128, 127] (torch.clamp passes the gradient inside the range,
+    bounds included), offsets rounded with a straight-through gradient, and an offset of
+    0 giving A = 0 at q = -128 (abs(-128) = -128 in the reference's int8 arithmetic)."""
     idx = idx or layer_index(dec)
     B = llr.shape[0]
     P = llr
@@ -33,7 +38,13 @@ def forward(dec, llr, alpha, beta, cap=1e30, idx=None):
         for l, (v, c) in enumerate(idx):
             w, Z = v.shape
             q = P[:, v.reshape(-1)].reshape(B, w, Z) - C[:, c.reshape(-1)].reshape(B, w, Z)
-            A = torch.relu(q.abs() - beta[t, l])
+            b = beta[t, l]
+            if int8:
+                q = torch.clamp(q, -128.0, 127.0)
+                b = b + (torch.round(b) - b).detach()
+            A = torch.relu(q.abs() - b)
+            if int8:
+                A = torch.where((b.detach() == 0) & (q.detach() == -128), torch.zeros_like(A), A)
             im = torch.argmin(A, dim=1, keepdim=True)                    # first minimum
             onehot = torch.zeros_like(A, dtype=torch.bool).scatter_(1, im, True)
             m1 = torch.gather(A, 1, im)
@@ -44,6 +55,8 @@ def forward(dec, llr, alpha, beta, cap=1e30, idx=None):
             neg = torch.logical_xor(sb, (sb.sum(dim=1, keepdim=True) % 2).bool())
             rn = torch.where(neg, -m, m)
             newp = q + rn
+            if int8:
+                newp = torch.clamp(newp, -128.0, 127.0)
             P = P.index_copy(1, v.reshape(-1), newp.reshape(B, -1))
             C = C.index_copy(1, c.reshape(-1), rn.reshape(B, -1))
     return P
