@@ -88,15 +88,23 @@ int fecsim_decoder_create(const fecsim_hdec *h, const char *kind, const fecsim_d
  * compiled for it (even Z <= 64, int8 min-sum, 1 <= beta, 0 <= cap, dense layout), the
  * code-specialized int8 kernel when the whole code was compiled in (5G NR base graph 1
  * at Z = 384; int8 min-sum, 1 <= beta, 0 <= cap, dense 16-byte aligned buffers, any
- * offset table with entries 1..127), and the generic kernel otherwise; all give
- * identical results.  No reference counterpart (the reference selects its decoder at
- * compile time only). */
-enum { FECSIM_KERNEL_AUTO = 0, FECSIM_KERNEL_GENERIC = 1, FECSIM_KERNEL_PAIR = 2, FECSIM_KERNEL_CODE = 3 };
+ * offset table with entries 1..127), the register-C2V float kernel for float32 decoders
+ * (min-sum or sum-product) whose code shape was compiled for it (802.11ad, 802.11n and
+ * 802.16e rate 1/2, Z <= 256, dense layout, any offset table), and the generic kernel
+ * otherwise; all give identical results.  No reference counterpart (the reference
+ * selects its decoder at compile time only). */
+enum {
+    FECSIM_KERNEL_AUTO = 0,
+    FECSIM_KERNEL_GENERIC = 1,
+    FECSIM_KERNEL_PAIR = 2,
+    FECSIM_KERNEL_CODE = 3,
+    FECSIM_KERNEL_FLOAT = 4
+};
 /* FECSIM_E_INVAL while a queue decoder (fecsim_multi_create) or a simulation loop
  * (fecsim_sim_create) holds d, as fecsim_decoder_set_offsets */
 int fecsim_decoder_select_kernel(fecsim_decoder *d, int kernel);
 /* kernel a dense decode_set call on aligned buffers would run now
- * (FECSIM_KERNEL_GENERIC, _PAIR or _CODE) */
+ * (FECSIM_KERNEL_GENERIC, _PAIR, _CODE or _FLOAT) */
 int fecsim_decoder_kernel(const fecsim_decoder *d);
 
 /* element kind (FECSIM_I8..F64), mode (FECSIM_MIN/PROD), bytes per element */

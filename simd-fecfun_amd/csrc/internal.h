@@ -207,6 +207,7 @@ struct DecoderDev {
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
   int pair_shape = -1;            // compiled code shape for the pair kernel, -1 if none
+  int freg_shape = -1;            // compiled code shape for the register-C2V float kernel, -1 if none
   PairLayout pair;
   uint32_t* d_pair_adr = nullptr;
   int kernel = FECSIM_KERNEL_AUTO;
@@ -220,6 +221,10 @@ struct DecoderDev {
 };
 
 int pair_i8_shape(const DecoderDev& d);
+// register-C2V float32 kernel (csrc/float_reg.hip): the compiled shape that runs decoder
+// d (float32, Z <= 256, shape compiled in), -1 if none; and its launch
+int float_reg_shape(const DecoderDev& d);
+hipError_t launch_float_reg(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
 // the row-pair kernel can run decoder d with an offset table of n entries
 bool pair_i8_table_ok(const DecoderDev& d, size_t n);
 void pair_layout(const HdecHost& h, PairLayout* pl);

@@ -86,6 +86,18 @@ inline bool pair_shape_fixed_z(int shape, int z, int mcol) {
   }
 }
 
+// the shape has a register-C2V float32 decoder
+inline bool float_shape_ok(int shape) {
+  switch (shape) {
+  case 0:
+  case 4:
+  case 7:
+  case 8:
+    return true;
+  default: return false;
+  }
+}
+
 struct PairArgs;
 hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st);
 

@@ -110,7 +110,7 @@ def config2(args):
         _log("config 2: z=%d %.3f s cwer %s; decode at 1.5 dB %.2f ms" % (z, dt, ["%.2e" % c for c in per_z[-1]["cwer"]],
                                                                           ms))
     return {"config": 2, "roofline": _roofline(
-                "k_decode<FloatPolicy<float,true>,8> (Z<=64) / k_decode_wg<...> (Z>=68)", roof_eu, roof_ms, OPS_SP,
+                "k_float_reg<FloatPolicy<float,true>,PairShape8> (16e 1/2, C2V in registers)", roof_eu, roof_ms, OPS_SP,
                 "decode-only launches of 2^18 cw at Eb/N0 1.5 dB, one per Z, summed", launches=len(zs)), "workload": "16e 1/2 Z=24..96 step 4, Vec16f_prod (float32 sum-product), max 10 iterations, "
                                      "Eb/N0 0.5..2.5 dB step 0.5, 2^18 cw per point, simulation loop per Z",
             "ebn0_db": ebn0, "cw_per_s": tot_cw / tot_t, "codewords": tot_cw, "seconds": tot_t, "per_z": per_z,

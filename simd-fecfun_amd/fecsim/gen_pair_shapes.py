@@ -1,5 +1,7 @@
-"""Generates csrc/pair_shapes.h and csrc/pair_shapes_dispatch.h: the code shapes (layer
-count and row weights) the row-pair int8 decoder (csrc/pair_i8.hip) is compiled for.
+"""Generates csrc/pair_shapes.h, csrc/pair_shapes_dispatch.h and
+csrc/float_shapes_dispatch.h: the code shapes (layer count and row weights) the
+row-pair int8 decoder (csrc/pair_i8.hip) and the register-C2V float decoder
+(csrc/float_reg.hip) are compiled for.
 
 A shape is the row-weight sequence of a base matrix; the shifts and columns stay run-time
 data, so one shape serves every lifting size, num_par and row relabeling of a code.
@@ -19,6 +21,22 @@ FIXED_Z_SETS = ["11ad 1/2", "11ad 13/16", "11ad 3/4", "11ad 5/8"]
 PAIR_SETS = ["11ad 1/2", "11ad 13/16", "11ad 3/4", "11ad 5/8", "11n 1/2 z54", "11n 2/3 z54", "11n 3/4 z54",
              "11n 1/2 z27", "16e 1/2", "16e 2/3A", "ITU G.hn 1/2 z14"]
 MAX_EDGES = 90
+# shapes the float32 decoder with C2V in registers is compiled for (the float workloads:
+# rate-1/2 codes with row weights <= 8; one C2V register per edge of a row's layers)
+FLOAT_SETS = ["11ad 1/2", "11n 1/2 z54", "11n 1/2 z27", "16e 1/2"]
+
+
+def float_shapes():
+    """indices into shapes() of the shapes compiled for the register-C2V float decoder"""
+    sh = shapes()
+    out = []
+    for ms in FLOAT_SETS:
+        hc, _, _ = codes.get_hmatrix(ms)
+        w = tuple(int((r >= 0).sum()) for r in hc)
+        i = [s for s, _, _ in sh].index(w)
+        if i not in out:
+            out.append(i)
+    return out
 
 
 def shapes():
@@ -53,7 +71,12 @@ def emit(csrc_dir: str) -> None:
     for i, (_, _, (zc, mc)) in enumerate(sh):
         if zc:
             h.append("  case %d: return z == %d && mcol == %d;" % (i, zc, mc))
-    h += ["  default: return false;", "  }", "}", "", "struct PairArgs;",
+    fl = float_shapes()
+    h += ["  default: return false;", "  }", "}", "",
+          "// the shape has a register-C2V float32 decoder",
+          "inline bool float_shape_ok(int shape) {", "  switch (shape) {"]
+    h += ["  case %d:" % i for i in fl]
+    h += ["    return true;" if fl else "", "  default: return false;", "  }", "}", "", "struct PairArgs;",
           "hipError_t pair_dispatch(int shape, const PairArgs& a, hipStream_t st);", "", "}  // namespace fecsim", ""]
     d = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
          "// included inside namespace fecsim by pair_i8.hip after FECSIM_PAIR_LAUNCH is defined",
@@ -63,6 +86,14 @@ def emit(csrc_dir: str) -> None:
     d += ["  default: return hipErrorInvalidValue;", "  }", "}", ""]
     _write(os.path.join(csrc_dir, "pair_shapes.h"), "\n".join(h))
     _write(os.path.join(csrc_dir, "pair_shapes_dispatch.h"), "\n".join(d))
+    f = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
+         "// included inside namespace fecsim by float_reg.hip after FECSIM_FREG_LAUNCH is defined",
+         "template <class P>", "hipError_t float_dispatch(int shape, const KernelArgs& a, hipStream_t st) {",
+         "  switch (shape) {"]
+    for i in fl:
+        f.append("  case %d: FECSIM_FREG_LAUNCH(P, PairShape%d)" % (i, i))
+    f += ["  default: return hipErrorInvalidValue;", "  }", "}", ""]
+    _write(os.path.join(csrc_dir, "float_shapes_dispatch.h"), "\n".join(f))
 
 
 def _write(path, text):

@@ -46,8 +46,9 @@ def test_gpu_offset_table_matches_oracle(kind, snr, cuda):
     want = O.decode_ideal(cs.dec, llr, elem=elem, beta=1.0, beta_table=tab)
     d = Decoder(Hdec.from_tables(cs.dec), kind, 10, 31.0, 1.0)
     d.set_offsets(tab)
-    # int8 tables with entries 0..127 run on the row-pair kernel, the others on the generic one
-    assert d.kernel == (L.KERNEL_PAIR if elem == O.I8 else L.KERNEL_GENERIC)
+    # int8 tables with entries 0..127 run on the row-pair kernel, float32 ones on the
+    # register-C2V float kernel, the others on the generic one
+    assert d.kernel == {O.I8: L.KERNEL_PAIR, O.F32: L.KERNEL_FLOAT}.get(elem, L.KERNEL_GENERIC)
     x = torch.from_numpy(llr).cuda()
     out = torch.empty_like(x)
     it = torch.zeros(400, dtype=torch.int32, device="cuda")
