@@ -405,10 +405,11 @@ void fecsim_msim_free(fecsim_msim *s);
  * (iteration, layer): C2V_j = sign * alpha * min(cap, min_{k != j} max(|q_k| - beta, 0)).
  * With alpha = 1 and beta = beta_offset it is the Vec16f_min / Vec8d_min layer update
  * (src/LDPCDec/decode_functions.cpp:11-103).  The forward keeps checkpoints
- * (posteriors and C2V) every K = ceil(sqrt(T)) iterations only; the adjoint recomputes
- * each segment of K iterations from its checkpoint, keeping that segment's V2C
- * messages, and sweeps it backwards.  Device memory per codeword: ceil(T/K) (N + E)
- * + (K + 2) E values (N = mcol z variables, E = total_hc_ones z edges). */
+ * (posteriors and C2V) only at the starts of ceil(T/K) segments, K = ceil(sqrt(T)),
+ * the last segment K iterations long; the adjoint recomputes each earlier segment from
+ * its checkpoint, keeping that segment's V2C messages, and sweeps it backwards.  Device
+ * memory per codeword: ceil(T/K) (N + E) + (K + 2) E values, plus 2 N when the
+ * posteriors sit in global memory (N = mcol z variables, E = total_hc_ones z edges). */
 typedef struct fecsim_trainer fecsim_trainer;
 
 typedef struct {

@@ -252,6 +252,8 @@ hipError_t launch_multi(int elem, int mode, int maxw, const MultiArgs& a, uint32
 int shape_with_rows(const HdecHost& h, int elem_bytes, int rows_per_lane, DecShape* s);
 // forward (bwd = 0) or adjoint + parameter-gradient sum (bwd = 1) of the trainable decoder
 hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t stream);
+// waves of that kernel (with a's global_p and waves) resident per CU (0: query failed)
+int train_resident_waves(const TrainArgs& a, int bwd);
 int encode_shape(int z, int mcol, EncArgs* a);
 hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);

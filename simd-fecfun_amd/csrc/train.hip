@@ -8,11 +8,13 @@
 // With alpha = 1 and a constant beta it equals the reference decoder on codewords
 // that need all T iterations.
 //
-// Memory: the forward keeps checkpoints only every K = ceil(sqrt(T)) iterations
-// (posteriors P and C2V C at the start of iterations 0, K, 2K, ...) and one working C2V
-// array.  The adjoint takes the segments [sK, sK + K) in reverse: it restarts from the
-// segment's checkpoint and runs the segment's iterations again, keeping their V2C
-// messages q (the only values the reverse sweep needs), then sweeps those iterations
+// Memory: the forward keeps checkpoints only at the starts of NS = ceil(T / K) segments,
+// K = ceil(sqrt(T)) (posteriors P and C2V C), and one working C2V array.  The segments
+// are [0, f), [f, f + K), ..., [T - K, T) with f = T - (NS - 1) K: the last one, whose
+// V2C messages the forward keeps, is a full K long, so the adjoint runs T - K
+// iterations again (not T - f).  The adjoint takes the segments in reverse: it restarts
+// from the segment's checkpoint and runs the segment's iterations again, keeping their
+// V2C messages q (the only values the reverse sweep needs), then sweeps those iterations
 // backwards, layer by layer:
 //   g_out_j = gP[v_j] + gC[e_j]           (C2V_j feeds the posterior and iteration t+1)
 //   g_m_j   = sign_j * g_out_j,  g_alpha += g_m_j * min(cap, x_j),  g_A[src_j] += alpha * g_m_j (x_j < cap)
@@ -196,6 +198,11 @@ __device__ __forceinline__ F offset_of(F b) {
   return b;
 }
 
+// first iteration of segment s: [0, f), [f, f + K), ... with f = T - (NS - 1) K
+__device__ __forceinline__ int seg_start(int s, int T, int K, int NS) {
+  return s == 0 ? 0 : T - (NS - s) * K;
+}
+
 template <typename F, bool GP, int MAXW, bool QT>
 __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   HIP_DYNAMIC_SHARED(char, smem);
@@ -221,13 +228,15 @@ __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
   for (int i = lane; i < N; i += 64) P[i] = llr[i];
   for (int i = lane; i < E; i += 64) C[i] = F(0);
   wave_sync();
-  const int tlast = (NS - 1) * K;   // the adjoint starts with this segment: its q are kept now
+  const int tlast = seg_start(NS - 1, T, K, NS);   // the adjoint starts with this segment: its q are kept now
+  const int f = seg_start(1, T, K, NS);           // first segment's length (T when NS = 1)
   F* Qseg = reinterpret_cast<F*>(a.qs) + cw * (uint64_t)K * E;
   for (int t = 0; t < T; ++t) {
     F* Qs = t >= tlast ? Qseg + (uint64_t)(t - tlast) * E : (F*)nullptr;
-    if (t % K == 0) {   // checkpoint at the start of segment t / K
-      F* cp = ckP + (uint64_t)(t / K) * N;
-      F* cc = ckC + (uint64_t)(t / K) * E;
+    const int s = t == 0 ? 0 : (t >= f && (t - f) % K == 0 ? (t - f) / K + 1 : -1);
+    if (s >= 0) {   // checkpoint at the start of segment s
+      F* cp = ckP + (uint64_t)s * N;
+      F* cc = ckC + (uint64_t)s * E;
       for (int i = lane; i < N; i += 64) cp[i] = P[i];
       for (int i = lane; i < E; i += 64) cc[i] = C[i];
     }
@@ -274,7 +283,7 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
   for (int i = lane; i < E; i += 64) gC[i] = F(0);
   wave_sync();
   for (int sg = NS - 1; sg >= 0; --sg) {
-    const int t0 = sg * K, t1 = (t0 + K < T) ? t0 + K : T;
+    const int t0 = seg_start(sg, T, K, NS), t1 = sg == NS - 1 ? T : seg_start(sg + 1, T, K, NS);
     // the segment's forward again, from its checkpoint, keeping every iteration's q (the
     // last segment's q were kept by the forward)
     if (sg < NS - 1) {
@@ -368,7 +377,34 @@ hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <typename F, bool GP, int MAXW>
+const void* kernel_tw(const TrainArgs& a, int bwd) {
+  if constexpr (std::is_same<F, float>::value) {
+    if (a.quant)
+      return bwd ? (const void*)k_train_bwd<F, GP, MAXW, true> : (const void*)k_train_fwd<F, GP, MAXW, true>;
+  }
+  return bwd ? (const void*)k_train_bwd<F, GP, MAXW, false> : (const void*)k_train_fwd<F, GP, MAXW, false>;
+}
+
+template <typename F, bool GP>
+const void* kernel_t(const TrainArgs& a, int bwd) {
+  return a.maxw <= 8    ? kernel_tw<F, GP, 8>(a, bwd)
+         : a.maxw <= 16 ? kernel_tw<F, GP, 16>(a, bwd)
+         : a.maxw <= 24 ? kernel_tw<F, GP, 24>(a, bwd)
+                        : kernel_tw<F, GP, kTrainMaxW>(a, bwd);
+}
+
 }  // namespace
+
+int train_resident_waves(const TrainArgs& a, int bwd) {
+  const void* k = a.is_double ? (a.global_p ? kernel_t<double, true>(a, bwd) : kernel_t<double, false>(a, bwd))
+                              : (a.global_p ? kernel_t<float, true>(a, bwd) : kernel_t<float, false>(a, bwd));
+  const size_t lds = (size_t)a.tab_bytes +
+                     (a.global_p ? 0 : (size_t)a.waves * 2 * a.n_pad * (a.is_double ? 8 : 4));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * a.waves, lds) != hipSuccess) return 0;
+  return per_cu * a.waves;
+}
 
 hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t st) {
   if (a.batch == 0) return hipSuccess;

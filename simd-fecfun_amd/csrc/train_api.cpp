@@ -13,7 +13,8 @@ struct fecsim_trainer {
   HdecHost h;
   fecsim_train_params p{};
   int elem_bytes = 4;
-  int waves = 4, global_p = 0, n_pad = 0, tab_bytes = 0;
+  int n_pad = 0, tab_bytes = 0;
+  int waves[2] = {4, 4}, global_p[2] = {0, 0};   // per kernel: [forward, backward]
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
   int seg = 1;   // checkpoint every seg iterations
@@ -50,7 +51,7 @@ void free_all(fecsim_trainer* t) {
   (void)hipFree(t->beta);
 }
 
-TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
+TrainArgs base_args(const fecsim_trainer* t, uint32_t batch, int bwd) {
   TrainArgs a{};
   const HdecHost& h = t->h;
   a.edges = t->d_edges;
@@ -64,10 +65,10 @@ TrainArgs base_args(const fecsim_trainer* t, uint32_t batch) {
   a.tab_bytes = t->tab_bytes;
   a.iters = (int)t->p.iters;
   a.maxw = h.max_row_weight;
-  a.waves = t->waves;
+  a.waves = t->waves[bwd];
   a.is_double = t->p.elem == FECSIM_F64;
   a.quant = t->p.elem == FECSIM_I8;
-  a.global_p = t->global_p;
+  a.global_p = t->global_p[bwd];
   a.cap = (double)t->p.cap;
   a.batch = batch;
   a.seg = t->seg;
@@ -102,12 +103,22 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   t->tab_bytes = ((h.total_hc_ones + h.mrow) * 4 + 15) / 16 * 16;   // edge words + layer weights in LDS
   const size_t wave_bytes = 2 * (size_t)t->n_pad * t->elem_bytes;
   const size_t room = 65536 - (size_t)t->tab_bytes;
+  // Posteriors and their adjoints in LDS (waves per block: the most waves a CU's 160 KB
+  // holds, ties to larger blocks) or in a global scratch slot per codeword; each kernel
+  // takes the form with more waves resident per CU.  For 802.11n z81 at T = 10 both take
+  // global posteriors: the adjoint (104 VGPRs) then holds 16 waves per CU against 10 with
+  // LDS and runs 11% faster (5.15 -> 4.57 ms per 4096 codewords); the forward is
+  // latency-bound either way (2.02 -> 2.06 ms).
+  int lds_waves = 0;
   if (wave_bytes <= room) {
-    t->waves = (int)(room / wave_bytes);
-    if (t->waves > 4) t->waves = 4;
-  } else {
-    t->global_p = 1;
-    t->waves = 4;
+    int best = 0;
+    for (int w = 1; w <= 4 && (size_t)w * wave_bytes <= room; ++w) {
+      const int resident = w * (int)(160 * 1024 / (t->tab_bytes + w * wave_bytes));
+      if (resident >= best) {
+        best = resident;
+        lds_waves = w;
+      }
+    }
   }
   std::vector<uint32_t> edges(h.total_hc_ones);
   for (int e = 0; e < h.total_hc_ones; ++e)
@@ -117,7 +128,8 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (he == hipSuccess) he = hipMalloc((void**)&t->d_layer_w, sizeof(int32_t) * h.mrow);
   if (he == hipSuccess) he = hipMemcpy(t->d_edges, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(t->d_layer_w, h.row_weights.data(), sizeof(int32_t) * h.mrow, hipMemcpyHostToDevice);
-  // checkpoints every K = ceil(sqrt(T)) iterations; the adjoint recomputes one segment of K
+  // checkpoints at the starts of ceil(T / K) segments, K = ceil(sqrt(T)), the last one K
+  // long; the adjoint recomputes one segment of at most K iterations at a time
   size_t K = 1;
   while (K * K < T) ++K;
   t->seg = (int)K;
@@ -128,7 +140,22 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (he == hipSuccess) he = hipMalloc(&t->qs, B * K * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->gc, B * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->partial, B * T * h.mrow * 2 * eb);
-  if (he == hipSuccess && t->global_p) he = hipMalloc(&t->gscratch, B * 2 * N * eb);
+  for (int bwd = 0; bwd < 2 && he == hipSuccess; ++bwd) {
+    t->global_p[bwd] = 1;
+    t->waves[bwd] = 4;
+    if (lds_waves) {
+      TrainArgs g = base_args(t, 1, bwd);
+      t->global_p[bwd] = 0;
+      t->waves[bwd] = lds_waves;
+      TrainArgs l = base_args(t, 1, bwd);
+      const int wg = train_resident_waves(g, bwd), wl = train_resident_waves(l, bwd);
+      if (wg > wl) {
+        t->global_p[bwd] = 1;
+        t->waves[bwd] = 4;
+      }
+    }
+  }
+  if (he == hipSuccess && (t->global_p[0] || t->global_p[1])) he = hipMalloc(&t->gscratch, B * 2 * N * eb);
   if (he == hipSuccess) he = hipMalloc(&t->alpha, T * h.mrow * eb);
   if (he == hipSuccess) he = hipMalloc(&t->beta, T * h.mrow * eb);
   if (he != hipSuccess) {
@@ -144,7 +171,7 @@ int fecsim_train_fwd(fecsim_trainer* t, uint32_t batch, const void* llr, const v
                      void* post, void* stream) {
   if (!t || ((!llr || !alpha || !beta || !post) && batch)) return fail(FECSIM_E_INVAL, "null argument");
   if (batch > t->p.max_batch) return fail(FECSIM_E_INVAL, "batch above max_batch");
-  TrainArgs a = base_args(t, batch);
+  TrainArgs a = base_args(t, batch, 0);
   const size_t pbytes = (size_t)t->p.iters * t->h.mrow * t->elem_bytes;
   hipError_t e = hipMemcpyAsync(t->alpha, alpha, pbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
   if (e == hipSuccess) e = hipMemcpyAsync(t->beta, beta, pbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
@@ -164,7 +191,7 @@ int fecsim_train_bwd(fecsim_trainer* t, uint32_t batch, const void* grad_post, v
                      void* grad_llr, void* stream) {
   if (!t || !grad_alpha || !grad_beta || (!grad_post && batch)) return fail(FECSIM_E_INVAL, "null argument");
   if (batch != t->last_batch || !t->have_fwd) return fail(FECSIM_E_INVAL, "backward needs a forward of the same batch since the last backward");
-  TrainArgs a = base_args(t, batch);
+  TrainArgs a = base_args(t, batch, 1);
   a.alpha = t->alpha;
   a.beta = t->beta;
   a.grad_post = grad_post;

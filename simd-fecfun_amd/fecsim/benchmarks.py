@@ -223,7 +223,7 @@ def config4(args):
     dt = time.perf_counter() - t0
     E = cs.dec.total_hc_ones * cs.z
     return {"config": 4, "workload": "11n 1/2 z81 trainable offset/normalized min-sum, T=10, batch 4096, "
-                                     "forward + adjoint (checkpoint per iteration, recomputation)",
+                                     "forward + adjoint (checkpoints every ceil(sqrt(T)) iterations, segments recomputed)",
             "steps": steps, "step_ms": dt / steps * 1e3, "forward_ms": fwd_ms / steps, "backward_ms": bwd_ms / steps,
             "cw_per_s_train": B * steps / dt, "edge_updates_per_s_forward": B * T * E / (fwd_ms / steps * 1e-3),
             "dtype": "f32", "data": "synthetic"}
