@@ -213,8 +213,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-offset CWER comparison")
     ap.add_argument("--learned-ncw", type=int, default=2_000_000, help="codewords per SNR point of that comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--kernel", choices=["auto", "generic", "pair"], default="auto",
-                    help="decoder kernel (results are identical)")
+    ap.add_argument("--kernel", choices=["auto", "generic", "pair", "pair_rt"], default="auto",
+                    help="decoder kernel (results are identical); pair_rt: the row-pair kernel's instance "
+                         "that reads the edge table at run time")
     ap.add_argument("--probe", choices=["traffic", "sim", "host"], default=None,
                     help="short fixed workloads for profiler runs instead of the benchmark: 'traffic' = "
                          "one decode of --ncw codewords at -0.5 dB twice, then a plain copy of the LLR "
@@ -248,8 +249,11 @@ def main():
     cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
     dec = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
     from fecsim import lib as _L
-    dec.select_kernel({"auto": _L.KERNEL_AUTO, "generic": _L.KERNEL_GENERIC, "pair": _L.KERNEL_PAIR}[args.kernel])
-    kname = "k_pair_i8<PairShape0> (row-pair, persistent)" if dec.kernel == _L.KERNEL_PAIR else "k_decode<IntPolicy<8>,8>"
+    dec.select_kernel({"auto": _L.KERNEL_AUTO, "generic": _L.KERNEL_GENERIC, "pair": _L.KERNEL_PAIR,
+                       "pair_rt": _L.KERNEL_PAIR_RT}[args.kernel])
+    kname = {_L.KERNEL_PAIR: "k_pair_i8c<PairShape0> (row-pair, edge table compiled in, persistent)",
+             _L.KERNEL_PAIR_RT: "k_pair_i8<PairShape0> (row-pair, run-time edge table, persistent)"}.get(
+                 dec.kernel, "k_decode<IntPolicy<8>,8>")
     ncw = args.ncw
     print("bench: generating %d x %d codeword LLRs on the GPU" % (len(SNRS), ncw), file=sys.stderr, flush=True)
     from fecsim.sim import Henc, Sim
@@ -373,7 +377,7 @@ def main():
         lt = time.perf_counter() - t0
         res["learned_table_decode"] = {
             "cw_per_s": ncw * len(SNRS) / lt, "info_gbit_s": ncw * len(SNRS) / lt * K_INFO / 1e9,
-            "kernel": "k_pair_i8<PairShape0> with the offset table" if ldec.kernel == _L.KERNEL_PAIR else "generic",
+            "kernel": "k_pair_i8c<PairShape0> with the offset table" if ldec.kernel == _L.KERNEL_PAIR else "generic",
             "note": "one pass over the five SNR batches with the learned per-(iteration, layer) offsets"}
         del lo
         print("bench: learned offsets", file=sys.stderr, flush=True)

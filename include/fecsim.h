@@ -98,13 +98,18 @@ enum {
     FECSIM_KERNEL_GENERIC = 1,
     FECSIM_KERNEL_PAIR = 2,
     FECSIM_KERNEL_CODE = 3,
-    FECSIM_KERNEL_FLOAT = 4
+    FECSIM_KERNEL_FLOAT = 4,
+    /* the row-pair kernel's instance that reads the edge table at run time, also for a
+     * code whose table is compiled into the row-pair kernel (the 802.11ad codes in the
+     * 64-lane int8 table order, which FECSIM_KERNEL_PAIR runs with the table compiled
+     * in); the same results */
+    FECSIM_KERNEL_PAIR_RT = 5
 };
 /* FECSIM_E_INVAL while a queue decoder (fecsim_multi_create) or a simulation loop
  * (fecsim_sim_create) holds d, as fecsim_decoder_set_offsets */
 int fecsim_decoder_select_kernel(fecsim_decoder *d, int kernel);
 /* kernel a dense decode_set call on aligned buffers would run now
- * (FECSIM_KERNEL_GENERIC, _PAIR, _CODE or _FLOAT) */
+ * (FECSIM_KERNEL_GENERIC, _PAIR, _CODE, _FLOAT, or _PAIR_RT when selected) */
 int fecsim_decoder_kernel(const fecsim_decoder *d);
 
 /* element kind (FECSIM_I8..F64), mode (FECSIM_MIN/PROD), bytes per element */

@@ -248,8 +248,8 @@ int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
   if (!d) return fail(FECSIM_E_INVAL, "null argument");
   if (d->holders.load() > 0)
     return fail(FECSIM_E_INVAL, "decoder is held by a queue decoder or simulation loop; select its kernel first");
-  if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_FLOAT) return fail(FECSIM_E_INVAL, "unknown kernel");
-  if (kernel == FECSIM_KERNEL_PAIR && d->d.pair_shape < 0)
+  if (kernel < FECSIM_KERNEL_AUTO || kernel > FECSIM_KERNEL_PAIR_RT) return fail(FECSIM_E_INVAL, "unknown kernel");
+  if ((kernel == FECSIM_KERNEL_PAIR || kernel == FECSIM_KERNEL_PAIR_RT) && d->d.pair_shape < 0)
     return fail(FECSIM_E_UNSUPPORTED, "no row-pair kernel for this code / element / parameters");
   if (kernel == FECSIM_KERNEL_CODE && d->d.code_tab < 0)
     return fail(FECSIM_E_UNSUPPORTED, "no code-specialized kernel for this code / element / parameters");
@@ -263,7 +263,8 @@ int fecsim_decoder_select_kernel(fecsim_decoder* d, int kernel) {
 // `aligned`: the buffers are 16-byte aligned (the code kernel moves 16-byte chunks)
 static int dense_kernel(const DecoderDev& d, bool aligned) {
   if (d.kernel == FECSIM_KERNEL_GENERIC) return FECSIM_KERNEL_GENERIC;
-  if (d.pair_shape >= 0 && (!d.d_beta_tab || d.d_beta_tab8)) return FECSIM_KERNEL_PAIR;
+  if (d.pair_shape >= 0 && (!d.d_beta_tab || d.d_beta_tab8))
+    return d.kernel == FECSIM_KERNEL_PAIR_RT ? FECSIM_KERNEL_PAIR_RT : FECSIM_KERNEL_PAIR;
   if (d.code_tab >= 0 && aligned && (!d.d_beta_tab || d.beta_tab_code_ok)) return FECSIM_KERNEL_CODE;
   if (d.freg_shape >= 0) return FECSIM_KERNEL_FLOAT;
   return FECSIM_KERNEL_GENERIC;
@@ -355,7 +356,8 @@ int fecsim_decode(fecsim_decoder* dec, uint32_t num_cw, int layout_p, const void
   a.tot_iter = tot_iter;
   const bool aligned = ((reinterpret_cast<uintptr_t>(llr_in) | reinterpret_cast<uintptr_t>(post_out)) & 15u) == 0;
   const int kern = layout_p == 0 ? dense_kernel(d, aligned) : FECSIM_KERNEL_GENERIC;
-  hipError_t he = kern == FECSIM_KERNEL_PAIR    ? launch_pair_i8(d, a, (hipStream_t)stream)
+  hipError_t he = kern == FECSIM_KERNEL_PAIR || kern == FECSIM_KERNEL_PAIR_RT
+                      ? launch_pair_i8(d, a, (hipStream_t)stream, kern == FECSIM_KERNEL_PAIR_RT)
                   : kern == FECSIM_KERNEL_CODE  ? launch_code_i8(d, a, (hipStream_t)stream)
                   : kern == FECSIM_KERNEL_FLOAT ? launch_float_reg(d, a, (hipStream_t)stream)
                                                 : launch_decode(d, a, (hipStream_t)stream);

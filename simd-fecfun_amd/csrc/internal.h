@@ -196,6 +196,7 @@ struct MultiArgs {
 // Layout of the row-pair int8 kernel (pair_i8.hip)
 struct PairLayout {
   int half = 0, cw_per_wave = 0, col_dwords = 0, wave_dwords = 0, waves = 0;
+  bool ct = false;                // the edge table is compiled into the shape's k_pair_i8c
   std::vector<uint32_t> adrtab;   // [half][total_hc_ones]
 };
 
@@ -228,7 +229,9 @@ hipError_t launch_float_reg(const DecoderDev& d, const KernelArgs& a, hipStream_
 // the row-pair kernel can run decoder d with an offset table of n entries
 bool pair_i8_table_ok(const DecoderDev& d, size_t n);
 void pair_layout(const HdecHost& h, PairLayout* pl);
-hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream);
+// run_time_table: the instance that reads the edge table at run time even where the
+// shape has it compiled in
+hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& a, hipStream_t stream, bool run_time_table);
 // code-specialized int8 kernel (one codeword per workgroup, code compiled in): the code's
 // index for decoder d, or -1; whether an offset table suits it; the launch
 int code_i8_index(const DecoderDev& d);

@@ -92,6 +92,7 @@ struct PairArgs {
   uint32_t* iters;
   int8_t* c2v;             // [num_cw][E] or null
   unsigned long long* tot_iter;
+  int ct;                  // the decoder's edge table is the shape's compiled-in one (k_pair_i8c)
 };
 
 // waves per SIMD the register allocation aims at: 4 when the edge addresses, the packed
@@ -384,6 +385,254 @@ k_pair_i8(const PairArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Instance with the code's edge table compiled in (fixed-Z shapes whose decoder table is
+// the P = 64 int8 one).  The same decode as k_pair_i8, bit for bit, with fewer registers
+// and VALU per edge:
+// * Edge addresses.  For an edge of shift s the lane reads pair t = (s + lr) mod Z, at
+//   position (c + lr) mod Z/2 of its column with c = s mod Z/2, in row
+//   [(s + lr) mod Z >= Z/2].  Per shift class c the lane keeps that address in both rows,
+//   ga[k] (row [c + lr >= Z/2]) and gb[k] = ga[k] ^ 256: an edge with s < Z/2 reads ga
+//   and writes its twin at gb, one with s >= Z/2 the other way round, and the column is
+//   an immediate offset.  2 registers per class (17 for 802.11ad 1/2) instead of one per
+//   edge (52), and no address arithmetic in the loop.
+// * C2V messages stay unpacked, one register per edge (both rows' messages as int16
+//   halves v << 8), subtracted and replaced as they are: no byte unpack or repack.
+// * The signed message is R = M * k, a 16-bit multiply by k = (Sa ^ sign Q_j) | 1 = +-1
+//   per half, Sa the sign mask of the row's XOR: (m << 8) * 0xffff = -(m << 8) mod 2^16.
+// * mv2 = min(cap, sat(m2 - beta)), which equals min(max(m2, beta), cap + beta) - beta for
+//   |Q| <= 128.
+template <class SH>
+constexpr int pair_c_waves_per_eu() {
+  return SH::ne + 2 * SH::ncls + 3 * SH::maxw + 20 <= 128 ? 4 : 1;
+}
+
+template <class SH, bool BT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(pair_c_waves_per_eu<SH>())))
+k_pair_i8c(const PairArgs a) {
+  constexpr int NE = SH::ne, ZC = SH::zc, L = ZC / 2, NC = SH::ncls, CPW = 64 / L;
+  static_assert(SH::ct && ZC % 2 == 0 && L <= 32, "compiled edge table, even Z <= 64");
+  HIP_DYNAMIC_SHARED(uint32_t, smem);
+  const int lane = threadIdx.x;              // one wave per block
+  const bool real = lane < CPW * L;          // lanes past the last slot mirror lane 0
+  const int slot = real ? lane / L : 0;
+  const int lr = real ? lane - slot * L : 0;
+  uint32_t* wbase = smem;
+  const int pos = lr + slot * L;             // this lane's pair position in a column row
+
+  // class addresses (absolute LDS byte addresses; the wave's columns start at 0)
+  uint32_t ga[NC], gb[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int t0 = SH::cls[k] + lr;          // < Z
+    const int o = t0 >= L;
+    ga[k] = (uint32_t)(o * 256 + (t0 - o * L + slot * L) * 4);
+    gb[k] = ga[k] ^ 256u;
+  }
+
+  const long long w0 = (long long)blockIdx.x * a.chunk;
+  long long w1 = w0 + a.chunk;
+  if (w1 > (long long)a.num_cw) w1 = a.num_cw;
+  long long next = w0;
+  long long cw = -1;
+  bool active = false;
+  uint32_t iter = 0;
+  uint32_t r[NE];                            // C2V of edge e: rows lr, lr + L as int16 halves v << 8
+#pragma unroll
+  for (int e = 0; e < NE; ++e) r[e] = 0u;
+  const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
+  const ushort2v cap2 = ushort2v{(unsigned short)(a.cap << 8), (unsigned short)(a.cap << 8)};
+  const unsigned long long gmask = (1ull << L) - 1ull;
+  unsigned long long my_iters = 0;
+  const uint8_t* btab = reinterpret_cast<const uint8_t*>(smem + (size_t)SH::mc * 128);
+  if constexpr (BT) {
+    uint8_t* bt = const_cast<uint8_t*>(btab);
+    const int nb = (int)a.max_iter * SH::mrow;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) bt[i] = (uint8_t)a.beta_tab[i];
+    __syncthreads();
+  }
+
+  for (;;) {
+    // ---- refill: idle slots take the next codewords of the block's range ----
+    const unsigned long long want = __ballot(real && lr == 0 && !active);
+    if (want != 0ull && next < w1) {
+      const int rank = __popcll(want & ((1ull << (slot * L)) - 1ull));
+      const long long mine = next + rank;
+      const bool take = !active && mine < w1;
+      next += __popcll(want);
+      if (take) {
+        cw = mine;
+        active = true;
+        iter = 0;
+        if (real) {
+          const uint8_t* g = reinterpret_cast<const uint8_t*>(a.llr_in) + cw * (long long)(SH::mc * ZC) + lr;
+          uint32_t* wp = wbase + pos;
+          static_assert(SH::mc % 4 == 0, "column groups of four");
+#pragma unroll
+          for (int c0 = 0; c0 < SH::mc; c0 += 4) {
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              lo[k] = g[(c0 + k) * ZC];
+              hi[k] = g[(c0 + k) * ZC + L];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              wp[(c0 + k) * 128] = __builtin_amdgcn_perm(hi[k], lo[k], 0x040c000cu);
+              wp[(c0 + k) * 128 + 64] = __builtin_amdgcn_perm(lo[k], hi[k], 0x040c000cu);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+      const uint32_t keep = take ? 0u : ~0u;   // fresh codeword: C2V messages start at zero
+#pragma unroll
+      for (int e = 0; e < NE; ++e) r[e] &= keep;
+    }
+    if (__ballot(active) == 0ull) break;
+    wave_sync();
+
+    // ---- one layered iteration; lanes of idle slots compute on their own (dead) slot ----
+#pragma unroll
+    for (int k = 0; k < NC; ++k) asm volatile("" : "+v"(ga[k]), "+v"(gb[k]));   // edge addresses formed at each use
+    int e0 = 0;
+    const uint8_t* bt_it = btab + (iter < a.max_iter ? iter : a.max_iter - 1u) * SH::mrow;
+#pragma unroll
+    for (int l = 0; l < SH::mrow; ++l) {
+      constexpr int MAXW = SH::maxw;
+      const int w = SH::w[l];
+      ushort2v beta2l = beta2;
+      uint32_t amask = 0xffffffffu;
+      if constexpr (BT) {   // the layer's offset b; an offset of 0 keeps abs(-128) = -128 (A = 0)
+        const uint32_t b = bt_it[l];
+        beta2l = as_us(b * 0x01000100u);
+        amask = b ? 0xffffffffu : 0x7fff7fffu;
+      }
+      short2v q[MAXW];
+      ushort2v A[MAXW];
+      uint32_t S = 0;
+      ushort2v m1 = ushort2v{0xffff, 0xffff}, m2 = ushort2v{0xffff, 0xffff};
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          const int e = e0 + j;
+          const uint32_t ad = (SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + SH::ecol[e] * 512;
+          const uint32_t x = *lds32(ad);
+          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(r[e]));
+          const short2v qq = as_s(as_u(qs) & 0xff00ff00u);   // Q << 8, Q = sat8(L - R)
+          q[j] = qq;
+          const ushort2v av = BT ? as_us(as_u(smax(qq, -qq)) & amask) : as_us(as_u(smax(qq, -qq)));
+          A[j] = av;
+          if (j & 1) S = xor3(S, as_u(q[j - 1]), as_u(qq));
+          else if (j + 1 == w) S ^= as_u(qq);
+          m2 = umin(m2, umax(m1, av));
+          m1 = umin(m1, av);
+        }
+      }
+      // capped exclusive minimum as in k_pair_i8: M_j = max(mv1, m1 + mv2 - |Q_j|)
+      const short2v mv1 = as_s(as_u(umin(cap2, __builtin_elementwise_sub_sat(m1, beta2l))));
+      const ushort2v mv2 = umin(cap2, __builtin_elementwise_sub_sat(m2, beta2l));
+      uint32_t sumb_u = as_u(m1 + mv2);
+      asm volatile("" : "+v"(sumb_u));   // one subtraction per edge, not re-associated into two
+      const short2v sumb = as_s(sumb_u);
+      const uint32_t sa = as_u(as_s(S) >> short2v{15, 15});   // 0 or 0xffff per half
+      uint32_t gu[MAXW];
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          const short2v m = smax(mv1, sumb - as_s(as_u(A[j])));   // argmin ? min2 : min1, capped
+          const uint32_t qa = as_u(q[j] >> short2v{15, 15});
+          const uint32_t k = __builtin_amdgcn_bitop3_b32(sa, qa, 0x00010001u, 0xBE);   // (sa ^ qa) | 1
+          const uint32_t rn = as_u(as_us(as_u(m)) * as_us(k));
+          gu[j] = as_u(__builtin_elementwise_add_sat(q[j], as_s(rn)));   // L = sat8(Q + R)
+          r[e0 + j] = rn;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j) {
+        if (j < w) {
+          const int e = e0 + j;
+          const uint32_t c = SH::ecol[e] * 512;
+          const uint32_t ad = (SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + c;
+          const uint32_t tw = (SH::eflip[e] ? ga[SH::ecls[e]] : gb[SH::ecls[e]]) + c;
+          *lds32(ad) = gu[j];
+          *lds32(tw) = __builtin_amdgcn_alignbit(gu[j], gu[j], 16);   // twin: halves swapped
+        }
+      }
+      e0 += w;
+      wave_sync();
+      __builtin_amdgcn_sched_barrier(0);   // schedule layer by layer: bounded live ranges
+    }
+    // ---- syndrome after the full iteration, LDPCdecode.cpp:172-195 ----
+#pragma unroll
+    for (int k = 0; k < NC; ++k) asm volatile("" : "+v"(ga[k]), "+v"(gb[k]));
+    uint32_t F = 0;
+    e0 = 0;
+#pragma unroll
+    for (int l = 0; l < SH::mrow; ++l) {
+      const int w = SH::w[l];
+      uint32_t xs[SH::maxw];
+#pragma unroll
+      for (int j = 0; j < SH::maxw; ++j)
+        if (j < w) {
+          const int e = e0 + j;
+          xs[j] = *lds32((SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + SH::ecol[e] * 512);
+        }
+      uint32_t par = xs[0];
+#pragma unroll
+      for (int j = 1; j + 1 < SH::maxw; j += 2)
+        if (j + 1 < w) par = xor3(par, xs[j], xs[j + 1]);
+      if (w & 1) F |= par;
+      else F = __builtin_amdgcn_bitop3_b32(F, par, xs[w - 1], 0xF6);   // F | (par ^ x)
+      e0 += w;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const bool fail = (F & 0x80008000u) != 0u;
+    const unsigned long long fb = __ballot(real && active && fail);
+    ++iter;
+    const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;
+    const bool finish = active && (!cwfail || iter == a.max_iter);   // LDPCdecode.cpp:192-197
+    if (finish && real) {
+      const uint32_t* wp = wbase + pos;
+      uint8_t* g = reinterpret_cast<uint8_t*>(a.post_out) + cw * (long long)(SH::mc * ZC) + lr;
+      static_assert(SH::mc % 8 == 0, "column groups of eight");
+#pragma unroll
+      for (int c0 = 0; c0 < SH::mc; c0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = wp[(c0 + k) * 128];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t x = v[k] >> 8;
+          g[(c0 + k) * ZC] = (uint8_t)x;
+          g[(c0 + k) * ZC + L] = (uint8_t)(x >> 16);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (a.c2v) {
+        int zo = ZC;
+        asm volatile("" : "+v"(zo));
+        int8_t* gc = a.c2v + cw * (long long)(NE * ZC) + lr;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          gc[e * zo] = (int8_t)(r[e] >> 8);
+          gc[e * zo + L] = (int8_t)(r[e] >> 24);
+        }
+      }
+      if (lr == 0) {
+        if (a.iters) a.iters[cw] = iter;
+        my_iters += iter;
+      }
+    }
+    active = active && !finish;
+  }
+  if (a.tot_iter) {
+    unsigned long long s = 0;
+    for (int q = 0; q < CPW; ++q) s += (unsigned long long)__shfl((long long)my_iters, q * L, 64);
+    if (lane == 0 && s) atomicAdd(a.tot_iter, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
 
 bool pair_i8_table_ok(const DecoderDev& d, size_t n) {
   return d.pair_shape >= 0 && n <= 4096 && pair_shape_fixed_z(d.pair_shape, d.h.z, d.h.mcol);
@@ -403,10 +652,18 @@ int pair_i8_shape(const DecoderDev& d) {
   return pair_shape_index(h.mrow, h.row_weights.data());
 }
 
-template <class SH, int ZC, bool BT>
+// CT: the instance with the code's edge table compiled in (k_pair_i8c)
+template <class SH, int ZC, bool BT, bool CT>
 static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   PairArgs a = in;
   if (a.num_cw == 0) return hipSuccess;
+  void (*kfn)(const PairArgs) = nullptr;
+  if constexpr (CT) {
+    if (a.waves != 1) return hipErrorInvalidValue;
+    kfn = k_pair_i8c<SH, BT>;
+  } else {
+    kfn = k_pair_i8<SH, ZC, BT>;
+  }
   const size_t lds = (size_t)a.waves * a.wave_dwords * 4 + (BT ? ((size_t)a.max_iter * SH::mrow + 3) / 4 * 4 : 0);
   // grid of 8x the resident blocks, each wave working through a contiguous chunk of
   // codewords with per-slot refill: chunks of ~30 codewords per wave keep the spread of
@@ -425,11 +682,11 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
       // the twin addressing (byte address xor 256) needs the waves' columns at multiples
       // of 512 bytes, i.e. the dynamic LDS starting at 0: no static LDS in the kernel
       hipFuncAttributes fa;
-      if (hipFuncGetAttributes(&fa, (const void*)k_pair_i8<SH, ZC, BT>) != hipSuccess || fa.sharedSizeBytes != 0)
+      if (hipFuncGetAttributes(&fa, (const void*)kfn) != hipSuccess || fa.sharedSizeBytes != 0)
         return hipErrorInvalidDeviceFunction;
       int cus = 0, per_cu = 0;
       if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_i8<SH, ZC, BT>, 64 * a.waves, lds) != hipSuccess)
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * a.waves, lds) != hipSuccess)
         return hipErrorInvalidValue;
       resident_by_dev[dev] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
     }
@@ -441,18 +698,25 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   if (blocks > cap_blocks) blocks = cap_blocks;
   const long long waves = blocks * a.waves;
   a.chunk = (a.num_cw + waves - 1) / waves;
-  hipLaunchKernelGGL((k_pair_i8<SH, ZC, BT>), dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
+  hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
   return hipGetLastError();
 }
 
 template <class SH>
 static hipError_t launch_shape(const PairArgs& a, hipStream_t st) {
   if constexpr (SH::zc != 0) {
-    if (a.z == SH::zc && a.mcol == SH::mc && a.col_dwords == 128)
-      return a.beta_tab ? launch_shape_z<SH, SH::zc, true>(a, st) : launch_shape_z<SH, SH::zc, false>(a, st);
+    if (a.z == SH::zc && a.mcol == SH::mc && a.col_dwords == 128) {
+      if constexpr (SH::ct) {
+        if (a.ct)
+          return a.beta_tab ? launch_shape_z<SH, SH::zc, true, true>(a, st)
+                            : launch_shape_z<SH, SH::zc, false, true>(a, st);
+      }
+      return a.beta_tab ? launch_shape_z<SH, SH::zc, true, false>(a, st)
+                        : launch_shape_z<SH, SH::zc, false, false>(a, st);
+    }
   }
   if (a.beta_tab) return hipErrorInvalidValue;   // tables run on the fixed-Z instances only
-  return launch_shape_z<SH, 0, false>(a, st);
+  return launch_shape_z<SH, 0, false, false>(a, st);
 }
 
 // Host-side layout of the pair kernel for decoder `d` (tables in d.h).
@@ -475,9 +739,12 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
         const int d = c * 128 + (t >= L ? 64 : 0) + (t % L) + slot * L;
         pl->adrtab[((size_t)slot * L + lr) * NE + e] = (uint32_t)(d * 4);
       }
+  const int shape = pair_shape_index(h.mrow, h.row_weights.data());
+  pl->ct = shape >= 0 && pair_shape_fixed_z(shape, h.z, h.mcol) &&
+           pair_shape_table(shape, NE, h.row_active_cols.data(), h.row_offset.data());
 }
 
-hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t st) {
+hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t st, bool run_time_table) {
   const HdecHost& h = d.h;
   const PairLayout& pl = d.pair;
   PairArgs a{};
@@ -499,6 +766,7 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   a.iters = ka.iters;
   a.c2v = reinterpret_cast<int8_t*>(ka.c2v);
   a.tot_iter = ka.tot_iter;
+  a.ct = pl.ct && !run_time_table ? 1 : 0;
   return pair_dispatch(d.pair_shape, a, st);
 }
 

@@ -103,7 +103,8 @@ class Decoder:
         self.np_dtype = NP_OF_ELEM[self.elem]
 
     def select_kernel(self, kernel: int) -> None:
-        """lib.KERNEL_AUTO / KERNEL_GENERIC / KERNEL_PAIR (results are identical)."""
+        """lib.KERNEL_AUTO / KERNEL_GENERIC / KERNEL_PAIR / KERNEL_CODE / KERNEL_FLOAT /
+        KERNEL_PAIR_RT (results are identical)."""
         _L.check(_L.lib().fecsim_decoder_select_kernel(self._d, kernel), "fecsim_decoder_select_kernel")
 
     def set_offsets(self, beta) -> None:

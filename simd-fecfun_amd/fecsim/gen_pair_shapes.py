@@ -5,6 +5,11 @@ row-pair int8 decoder (csrc/pair_i8.hip) and the register-C2V float decoder
 
 A shape is the row-weight sequence of a base matrix; the shifts and columns stay run-time
 data, so one shape serves every lifting size, num_par and row relabeling of a code.
+The fixed-Z shapes also carry their code's edge table (column and shift of every edge,
+as gen_structs writes it for P = 64 lanes of int8, the LDPCtop_Vec64c_min layout):
+a decoder whose tables equal it runs the kernel instance with the table compiled in
+(edge addresses from one register per shift class and orientation, columns as
+immediate offsets); other tables of the same shape take the run-time-table instance.
 Run by simd-fecfun_amd/build.py before compiling.
 """
 from __future__ import annotations
@@ -39,6 +44,24 @@ def float_shapes():
     return out
 
 
+def edge_table(ms: str):
+    """(columns, shifts) of every edge of matrix set ms in the decoder-table order that
+    gen_structs writes for 64 int8 lanes (build/gen_structs.m with simd 512 / elem 8)"""
+    d = codes.gen_structs(ms, simd_size=512, elem_size=8).dec
+    return [int(c) for c in d.row_active_cols], [int(x) for x in d.row_offset]
+
+
+def shift_classes(shifts, z):
+    """the distinct shift classes s mod Z/2 in order of first use, and per edge its class
+    index and whether s >= Z/2 (the pair it reads is the twin of the class's pair)"""
+    half = z // 2
+    cls = []
+    for x in shifts:
+        if x % half not in cls:
+            cls.append(x % half)
+    return cls, [cls.index(x % half) for x in shifts], [int(x >= half) for x in shifts]
+
+
 def shapes():
     """[(row weights, matrix set, (Z, mcol) compiled in or (0, 0))]"""
     out = []
@@ -60,6 +83,20 @@ def emit(csrc_dir: str) -> None:
         h.append("  static constexpr int mrow = %d, ne = %d, maxw = %d;" % (len(w), sum(w), max(w)))
         h.append("  static constexpr int zc = %d, mc = %d;  // Z and columns of the fixed-Z kernel (0: none)" % (zc, mc))
         h.append("  static constexpr int w[%d] = {%s};" % (len(w), ", ".join(map(str, w))))
+        if zc:
+            col, sft = edge_table(ms)
+            cls, ecls, eflip = shift_classes(sft, zc)
+            h.append("  // edge table compiled in (P = 64 int8 layout): column, shift, shift class "
+                     "(shift mod Z/2) and shift >= Z/2 per edge")
+            h.append("  static constexpr bool ct = true;")
+            h.append("  static constexpr int ncls = %d;" % len(cls))
+            for nm, arr in (("ecol", col), ("esft", sft), ("ecls", ecls), ("eflip", eflip)):
+                h.append("  static constexpr int %s[%d] = {%s};" % (nm, len(arr), ", ".join(map(str, arr))))
+            h.append("  static constexpr int cls[%d] = {%s};" % (len(cls), ", ".join(map(str, cls))))
+        else:
+            h.append("  static constexpr bool ct = false;")
+            h.append("  static constexpr int ncls = 0;")
+            h.append("  static constexpr int ecol[1] = {0}, esft[1] = {0}, ecls[1] = {0}, eflip[1] = {0}, cls[1] = {0};")
         h.append("};")
     h += ["", "inline int pair_shape_index(int mrow, const int32_t* w) {"]
     for i, (w, _, _) in enumerate(sh):
@@ -71,6 +108,21 @@ def emit(csrc_dir: str) -> None:
     for i, (_, _, (zc, mc)) in enumerate(sh):
         if zc:
             h.append("  case %d: return z == %d && mcol == %d;" % (i, zc, mc))
+    h += ["  default: return false;", "  }", "}", "",
+          "// the shape's compiled-in edge table equals (cols, shifts) of a decoder",
+          "inline bool pair_shape_table(int shape, int ne, const int32_t* cols, const int32_t* shifts) {",
+          "  switch (shape) {"]
+    for i, (w, ms, (zc, mc)) in enumerate(sh):
+        if zc:
+            col, sft = edge_table(ms)
+            h.append("  case %d: {" % i)
+            h.append("    static const int32_t c[%d] = {%s};" % (len(col), ", ".join(map(str, col))))
+            h.append("    static const int32_t s[%d] = {%s};" % (len(sft), ", ".join(map(str, sft))))
+            h.append("    if (ne != %d) return false;" % len(col))
+            h.append("    for (int e = 0; e < ne; ++e)")
+            h.append("      if (cols[e] != c[e] || shifts[e] != s[e]) return false;")
+            h.append("    return true;")
+            h.append("  }")
     fl = float_shapes()
     h += ["  default: return false;", "  }", "}", "",
           "// the shape has a register-C2V float32 decoder",

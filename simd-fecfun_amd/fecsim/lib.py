@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libfecsim.so")
 
 I8, I16, I32, F32, F64 = 0, 1, 2, 3, 4
 MIN, PROD = 0, 1
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_PAIR, KERNEL_CODE, KERNEL_FLOAT = 0, 1, 2, 3, 4
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_PAIR, KERNEL_CODE, KERNEL_FLOAT, KERNEL_PAIR_RT = 0, 1, 2, 3, 4, 5
 
 # (name, restype, argtypes) of every exported entry point
 _vp, _u32, _i32, _int, _u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_int, C.c_uint64

@@ -226,7 +226,7 @@ def test_gpu_pair_kernel_matches_oracle(case, cuda):
     llr[: max(1, ncw // 10)] = rng.integers(-128, 128, size=(max(1, ncw // 10), cs.n))   # saturation corners
     ref = O.decode_ideal(cs.dec, llr, max_row_metric=cap, beta=beta)
     h = Hdec.from_tables(cs.dec)
-    for kern in (L.KERNEL_PAIR, L.KERNEL_GENERIC):
+    for kern in (L.KERNEL_PAIR, L.KERNEL_PAIR_RT, L.KERNEL_GENERIC):
         d = Decoder(h, "Vec64c_min", 10, cap, beta)
         d.select_kernel(kern)
         assert d.kernel == kern
@@ -259,17 +259,19 @@ def test_gpu_pair_kernel_extreme_rows(ms, cuda):
     for beta, cap in [(1.0, 31.0), (1.0, 127.0), (2.0, 127.0), (3.0, 5.0), (127.0, 0.5), (64.0, 100.0)]:
         _say("extreme rows", ms, beta, cap)
         ref = O.decode_ideal(cs.dec, llr, max_iter=6, max_row_metric=cap, beta=beta)
-        d = Decoder(h, "Vec64c_min", 6, cap, beta)
-        assert d.kernel == L.KERNEL_PAIR
-        x = torch.from_numpy(llr).cuda()
-        out = torch.empty_like(x)
-        iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
-        c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
-        d.decode(x, out, iters=iters, c2v=c2v)
-        torch.cuda.synchronize()
-        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1]), (beta, cap)
-        assert np.array_equal(out.cpu().numpy(), ref[0]), (beta, cap)
-        assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2]), (beta, cap)
+        for kern in (L.KERNEL_PAIR, L.KERNEL_PAIR_RT):
+            d = Decoder(h, "Vec64c_min", 6, cap, beta)
+            assert d.kernel == L.KERNEL_PAIR
+            d.select_kernel(kern)
+            x = torch.from_numpy(llr).cuda()
+            out = torch.empty_like(x)
+            iters = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+            c2v = torch.zeros(ncw * h.e, dtype=torch.int8, device="cuda")
+            d.decode(x, out, iters=iters, c2v=c2v)
+            torch.cuda.synchronize()
+            assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), ref[1]), (kern, beta, cap)
+            assert np.array_equal(out.cpu().numpy(), ref[0]), (kern, beta, cap)
+            assert np.array_equal(iters.cpu().numpy().astype(np.uint32), ref[2]), (kern, beta, cap)
 
 
 def test_gpu_pair_kernel_selected_for_11ad(cuda):
@@ -467,9 +469,10 @@ def test_gpu_global_c2v_codes_match_oracle(case, cuda):
                          ids=lambda c: "snr%g-it%d-cap%g-n%d" % c)
 def test_gpu_pair_kernel_offset_tables(case, cuda):
     """Per-(iteration, layer) offset tables on the row-pair kernel (802.11ad rate 1/2,
-    entries 1..4): posteriors, C2V and iteration counts equal the oracle's with the same
-    table; the same decoder with a constant table equals the plain decoder; entries of 0
-    stay on the pair kernel, a negative entry falls back to the generic kernel."""
+    entries 1..4), both instances (edge table compiled in, and read at run time):
+    posteriors, C2V and iteration counts equal the oracle's with the same table; the same
+    decoder with a constant table equals the plain decoder; entries of 0 stay on the pair
+    kernel, a negative entry falls back to the generic kernel."""
     import torch
     from fecsim import lib as L
     snr, mi, cap, ncw = case
@@ -483,41 +486,43 @@ def test_gpu_pair_kernel_offset_tables(case, cuda):
     llr[1] = np.where(rng.random(cs.n) < 0.5, -128, 127)
     tab = rng.integers(1, 5, (mi, cs.dec.mrow)).astype(np.float32)
     want = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab)
-    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, cap, 1.0)
-    d.set_offsets(tab)
-    assert d.kernel == L.KERNEL_PAIR
-    x = torch.from_numpy(llr).cuda()
-    out = torch.empty_like(x)
-    it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
-    c2v = torch.zeros(ncw * cs.dec.total_hc_ones * cs.z, dtype=torch.int8, device="cuda")
-    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
-    d.decode(x, out, iters=it, c2v=c2v, tot_iter=tot)
-    torch.cuda.synchronize()
-    assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2])
-    assert np.array_equal(out.cpu().numpy(), want[0])
-    assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want[1])
-    assert int(tot.item()) == want[3]
-    # a constant table of 1 is the plain beta-1 decoder
-    d.set_offsets(np.ones((mi, cs.dec.mrow), np.float32))
-    assert d.kernel == L.KERNEL_PAIR
     plain = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0)
-    d.decode(x, out, iters=it)
-    torch.cuda.synchronize()
-    assert np.array_equal(out.cpu().numpy(), plain[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), plain[2])
-    # entries of 0 (the abs(-128) wrap path: a saturated -128 gets magnitude 0) on the pair
-    # kernel; a negative entry falls back to the generic kernel; both equal the oracle
-    for bad, kern in ((0.0, L.KERNEL_PAIR), (-1.0, L.KERNEL_GENERIC)):
-        tab0 = tab.copy()
-        tab0[0, 3] = bad
-        tab0[mi - 1, :] = bad
-        d.set_offsets(tab0)
-        assert d.kernel == kern
-        want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab0)
-        d.decode(x, out, iters=it, c2v=c2v)
+    for inst in (L.KERNEL_PAIR, L.KERNEL_PAIR_RT):
+        d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, cap, 1.0)
+        d.select_kernel(inst)
+        d.set_offsets(tab)
+        assert d.kernel == inst
+        x = torch.from_numpy(llr).cuda()
+        out = torch.empty_like(x)
+        it = torch.zeros(ncw, dtype=torch.int32, device="cuda")
+        c2v = torch.zeros(ncw * cs.dec.total_hc_ones * cs.z, dtype=torch.int8, device="cuda")
+        tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+        d.decode(x, out, iters=it, c2v=c2v, tot_iter=tot)
         torch.cuda.synchronize()
-        assert np.array_equal(out.cpu().numpy(), want0[0])
-        assert np.array_equal(it.cpu().numpy().astype(np.uint32), want0[2])
-        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want0[1])
+        assert np.array_equal(it.cpu().numpy().astype(np.uint32), want[2]), inst
+        assert np.array_equal(out.cpu().numpy(), want[0]), inst
+        assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want[1]), inst
+        assert int(tot.item()) == want[3]
+        # a constant table of 1 is the plain beta-1 decoder
+        d.set_offsets(np.ones((mi, cs.dec.mrow), np.float32))
+        assert d.kernel == inst
+        d.decode(x, out, iters=it)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), plain[0]) and np.array_equal(it.cpu().numpy().astype(np.uint32), plain[2])
+        # entries of 0 (the abs(-128) wrap path: a saturated -128 gets magnitude 0) on the
+        # pair kernel; a negative entry falls back to the generic kernel; both equal the oracle
+        for bad, kern in ((0.0, inst), (-1.0, L.KERNEL_GENERIC)):
+            tab0 = tab.copy()
+            tab0[0, 3] = bad
+            tab0[mi - 1, :] = bad
+            d.set_offsets(tab0)
+            assert d.kernel == kern
+            want0 = O.decode_ideal(cs.dec, llr, max_iter=mi, max_row_metric=cap, beta=1.0, beta_table=tab0)
+            d.decode(x, out, iters=it, c2v=c2v)
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), want0[0]), (inst, bad)
+            assert np.array_equal(it.cpu().numpy().astype(np.uint32), want0[2]), (inst, bad)
+            assert np.array_equal(c2v.cpu().numpy().reshape(ncw, -1), want0[1]), (inst, bad)
 
 
 def test_gpu_pair_kernel_table_lifted_z_falls_back(cuda):
