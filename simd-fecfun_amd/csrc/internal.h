@@ -197,7 +197,8 @@ struct MultiArgs {
 struct PairLayout {
   int half = 0, cw_per_wave = 0, col_dwords = 0, wave_dwords = 0, waves = 0;
   bool ct = false;                // the edge table is compiled into the shape's k_pair_i8c
-  std::vector<uint32_t> adrtab;   // [half][total_hc_ones]
+  size_t ct_off = 0;              // k_pair_i8c's per-lane table in adrtab (when ct)
+  std::vector<uint32_t> adrtab;   // [half][total_hc_ones], then k_pair_i8c's table
 };
 
 struct DecoderDev {

@@ -409,25 +409,29 @@ constexpr int pair_c_waves_per_eu() {
 template <class SH, bool BT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(pair_c_waves_per_eu<SH>())))
 k_pair_i8c(const PairArgs a) {
-  constexpr int NE = SH::ne, ZC = SH::zc, L = ZC / 2, NC = SH::ncls, CPW = 64 / L;
+  constexpr int NE = SH::ne, ZC = SH::zc, L = ZC / 2, NC = SH::ncls;
   static_assert(SH::ct && ZC % 2 == 0 && L <= 32, "compiled edge table, even Z <= 64");
   HIP_DYNAMIC_SHARED(uint32_t, smem);
   const int lane = threadIdx.x;              // one wave per block
-  const bool real = lane < CPW * L;          // lanes past the last slot mirror lane 0
-  const int slot = real ? lane / L : 0;
-  const int lr = real ? lane - slot * L : 0;
-  uint32_t* wbase = smem;
-  const int pos = lr + slot * L;             // this lane's pair position in a column row
-
-  // class addresses (absolute LDS byte addresses; the wave's columns start at 0)
+  // per-lane layout from pair_layout (a.adrtab): the class addresses (absolute LDS byte
+  // addresses, the wave's columns start at 0), the lane's pair position in a column row,
+  // its row lr, the lane of its slot's row 0 and the lanes of its slot.  Lanes, slots and
+  // pair positions are placed to keep LDS bank conflicts low; the spare lane mirrors a
+  // lane of its own half-wave (same addresses, same values: a broadcast)
+  const uint32_t* lt = a.adrtab + (size_t)lane * (NC + 4);
   uint32_t ga[NC], gb[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const int t0 = SH::cls[k] + lr;          // < Z
-    const int o = t0 >= L;
-    ga[k] = (uint32_t)(o * 256 + (t0 - o * L + slot * L) * 4);
+    ga[k] = lt[k];
     gb[k] = ga[k] ^ 256u;
   }
+  const int pos = (int)lt[NC];
+  const uint32_t info = lt[NC + 1];
+  const int lr = (int)(info & 0xffu);
+  const int lead = (int)((info >> 8) & 0xffu);
+  const bool real = ((info >> 16) & 1u) != 0u;
+  const unsigned long long smask = (unsigned long long)lt[NC + 2] | ((unsigned long long)lt[NC + 3] << 32);
+  uint32_t* wbase = smem;
 
   const long long w0 = (long long)blockIdx.x * a.chunk;
   long long w1 = w0 + a.chunk;
@@ -441,7 +445,6 @@ k_pair_i8c(const PairArgs a) {
   for (int e = 0; e < NE; ++e) r[e] = 0u;
   const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
   const ushort2v cap2 = ushort2v{(unsigned short)(a.cap << 8), (unsigned short)(a.cap << 8)};
-  const unsigned long long gmask = (1ull << L) - 1ull;
   unsigned long long my_iters = 0;
   const uint8_t* btab = reinterpret_cast<const uint8_t*>(smem + (size_t)SH::mc * 128);
   if constexpr (BT) {
@@ -453,9 +456,9 @@ k_pair_i8c(const PairArgs a) {
 
   for (;;) {
     // ---- refill: idle slots take the next codewords of the block's range ----
-    const unsigned long long want = __ballot(real && lr == 0 && !active);
+    const unsigned long long want = __ballot(real && lane == lead && !active);
     if (want != 0ull && next < w1) {
-      const int rank = __popcll(want & ((1ull << (slot * L)) - 1ull));
+      const int rank = __popcll(want & ((1ull << lead) - 1ull));
       const long long mine = next + rank;
       const bool take = !active && mine < w1;
       next += __popcll(want);
@@ -589,7 +592,7 @@ k_pair_i8c(const PairArgs a) {
     const bool fail = (F & 0x80008000u) != 0u;
     const unsigned long long fb = __ballot(real && active && fail);
     ++iter;
-    const bool cwfail = ((fb >> (slot * L)) & gmask) != 0ull;
+    const bool cwfail = (fb & smask) != 0ull;
     const bool finish = active && (!cwfail || iter == a.max_iter);   // LDPCdecode.cpp:192-197
     if (finish && real) {
       const uint32_t* wp = wbase + pos;
@@ -625,10 +628,11 @@ k_pair_i8c(const PairArgs a) {
     }
     active = active && !finish;
   }
-  if (a.tot_iter) {
-    unsigned long long s = 0;
-    for (int q = 0; q < CPW; ++q) s += (unsigned long long)__shfl((long long)my_iters, q * L, 64);
-    if (lane == 0 && s) atomicAdd(a.tot_iter, s);
+  if (a.tot_iter) {   // my_iters is nonzero on the slots' row-0 lanes only
+    long long s = (long long)my_iters;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0 && s) atomicAdd(a.tot_iter, (unsigned long long)s);
   }
 }
 
@@ -742,13 +746,70 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
   const int shape = pair_shape_index(h.mrow, h.row_weights.data());
   pl->ct = shape >= 0 && pair_shape_fixed_z(shape, h.z, h.mcol) &&
            pair_shape_table(shape, NE, h.row_active_cols.data(), h.row_offset.data());
+  pl->ct_off = pl->adrtab.size();
+  if (!pl->ct) return;
+  // k_pair_i8c's per-lane table, behind the run-time table: per hardware lane, the
+  // addresses of the shift classes (distinct s mod Z/2 in order of first use, as the shape
+  // header lists them) in row [c + lr >= Z/2], the pair position, lr | lead lane << 8 |
+  // real << 16, and the 64-bit mask of the slot's lanes
+  std::vector<int> cls;
+  for (int e = 0; e < NE; ++e) {
+    const int c = h.row_offset[e] % L;
+    bool seen = false;
+    for (int x : cls) seen = seen || x == c;
+    if (!seen) cls.push_back(c);
+  }
+  const int nc = (int)cls.size(), cpw = pl->cw_per_wave;
+  // lane -> (slot, lr) item and (slot, position) -> dword: the shape's bank-conflict
+  // layout when it has one, else slots in lane order and positions in slot order
+  int32_t lane_item[64], dword[64];
+  if (!pair_shape_lds_layout(shape, lane_item, dword)) {
+    for (int x = 0; x < 64; ++x) lane_item[x] = x < cpw * L ? x : -1;
+    for (int x = 0; x < cpw * L; ++x) dword[x] = x;
+  }
+  // a spare lane mirrors the first real lane of its own half-wave
+  int item_of[64];
+  bool real[64];
+  for (int x = 0; x < 64; ++x) {
+    real[x] = lane_item[x] >= 0;
+    item_of[x] = lane_item[x];
+    if (!real[x])
+      for (int y = x & 32; y < (x & 32) + 32; ++y)
+        if (lane_item[y] >= 0) {
+          item_of[x] = lane_item[y];
+          break;
+        }
+  }
+  int lead_of_slot[64] = {};
+  unsigned long long mask_of_slot[64] = {};
+  for (int x = 0; x < 64; ++x)
+    if (real[x]) {
+      const int sl = item_of[x] / L;
+      if (item_of[x] % L == 0) lead_of_slot[sl] = x;
+      mask_of_slot[sl] |= 1ull << x;
+    }
+  const size_t stride = (size_t)nc + 4;
+  pl->adrtab.resize(pl->ct_off + 64 * stride);
+  uint32_t* t = pl->adrtab.data() + pl->ct_off;
+  for (int x = 0; x < 64; ++x) {
+    const int sl = item_of[x] / L, lr = item_of[x] % L;
+    for (int k = 0; k < nc; ++k) {
+      const int t0 = cls[k] + lr, o = t0 >= L;
+      t[x * stride + k] = (uint32_t)(o * 256 + 4 * dword[sl * L + t0 - o * L]);
+    }
+    t[x * stride + nc] = (uint32_t)dword[sl * L + lr];
+    t[x * stride + nc + 1] = (uint32_t)lr | ((uint32_t)lead_of_slot[sl] << 8) | ((real[x] ? 1u : 0u) << 16);
+    t[x * stride + nc + 2] = (uint32_t)mask_of_slot[sl];
+    t[x * stride + nc + 3] = (uint32_t)(mask_of_slot[sl] >> 32);
+  }
 }
 
 hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t st, bool run_time_table) {
   const HdecHost& h = d.h;
   const PairLayout& pl = d.pair;
   PairArgs a{};
-  a.adrtab = d.d_pair_adr;
+  a.ct = pl.ct && !run_time_table ? 1 : 0;
+  a.adrtab = d.d_pair_adr + (a.ct ? pl.ct_off : 0);
   a.z = h.z;
   a.half = pl.half;
   a.mcol = h.mcol;
@@ -766,7 +827,6 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   a.iters = ka.iters;
   a.c2v = reinterpret_cast<int8_t*>(ka.c2v);
   a.tot_iter = ka.tot_iter;
-  a.ct = pl.ct && !run_time_table ? 1 : 0;
   return pair_dispatch(d.pair_shape, a, st);
 }
 

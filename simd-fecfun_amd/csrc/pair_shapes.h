@@ -178,6 +178,43 @@ inline bool pair_shape_table(int shape, int ne, const int32_t* cols, const int32
   }
 }
 
+// k_pair_i8c's LDS layout of the shape (lds_layout.py): per hardware lane the
+// (slot, lr) item slot * Z/2 + lr (-1: spare lane), per (slot, position) item its
+// dword in a column row; false: slots in lane order, positions in slot order
+inline bool pair_shape_lds_layout(int shape, int32_t* lane_item, int32_t* dword) {
+  switch (shape) {
+  case 0: {  // 11ad 1/2: 2.346 LDS cycles per access (3.808 in lane order)
+    static const int32_t li[64] = {0, 1, 8, 9, 10, 11, 12, 18, 19, 20, 26, 27, 28, 29, 30, 31, 37, 38, 39, 40, 41, 42, 43, 49, 50, 51, 52, 53, 59, 60, 61, 62, 2, 3, 4, 5, 6, 7, 13, 14, 15, 16, 17, 21, 22, 23, 24, 25, 32, 33, 34, 35, 36, 44, 45, 46, 47, 48, 54, 55, 56, 57, 58, -1};
+    static const int32_t dw[63] = {8, 39, 10, 5, 2, 54, 43, 38, 52, 3, 28, 9, 6, 23, 57, 47, 50, 24, 53, 19, 4, 62, 46, 26, 40, 7, 30, 31, 34, 13, 1, 16, 0, 51, 60, 41, 48, 55, 37, 15, 27, 56, 11, 49, 20, 35, 61, 59, 17, 32, 25, 29, 18, 44, 21, 63, 36, 45, 12, 42, 14, 58, 22};
+    for (int x = 0; x < 64; ++x) lane_item[x] = li[x];
+    for (int x = 0; x < 63; ++x) dword[x] = dw[x];
+    return true;
+  }
+  case 1: {  // 11ad 13/16: 2.311 LDS cycles per access (3.733 in lane order)
+    static const int32_t li[64] = {2, 3, 4, 5, 6, 8, 9, 10, 11, 21, 22, 23, 34, 35, 36, 37, 38, 39, 40, 41, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 61, 0, 1, 7, 12, 13, 14, 15, 16, 17, 18, 19, 20, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 42, 43, 44, 56, 57, 58, 59, 60, 62, -1};
+    static const int32_t dw[63] = {2, 28, 41, 3, 59, 50, 7, 48, 36, 10, 31, 46, 62, 1, 54, 0, 23, 15, 17, 24, 6, 38, 5, 40, 58, 13, 43, 25, 21, 51, 44, 20, 37, 8, 26, 45, 11, 29, 57, 53, 19, 12, 42, 63, 14, 30, 33, 22, 32, 55, 47, 49, 56, 52, 34, 60, 9, 35, 27, 18, 39, 16, 4};
+    for (int x = 0; x < 64; ++x) lane_item[x] = li[x];
+    for (int x = 0; x < 63; ++x) dword[x] = dw[x];
+    return true;
+  }
+  case 2: {  // 11ad 3/4: 2.339 LDS cycles per access (3.679 in lane order)
+    static const int32_t li[64] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 25, 27, 29, 31, 33, 35, 37, 39, 41, 21, 23, 26, 28, 30, 32, 34, 36, 38, 40, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 60, 61, 62, -1};
+    static const int32_t dw[63] = {31, 39, 22, 40, 50, 6, 45, 56, 15, 52, 62, 4, 25, 27, 60, 49, 23, 14, 34, 41, 12, 61, 1, 35, 3, 42, 10, 37, 5, 0, 32, 26, 58, 11, 43, 53, 16, 48, 51, 19, 33, 29, 24, 28, 13, 36, 55, 8, 38, 9, 46, 7, 44, 2, 17, 20, 18, 54, 59, 63, 30, 47, 57};
+    for (int x = 0; x < 64; ++x) lane_item[x] = li[x];
+    for (int x = 0; x < 63; ++x) dword[x] = dw[x];
+    return true;
+  }
+  case 3: {  // 11ad 5/8: 2.480 LDS cycles per access (3.740 in lane order)
+    static const int32_t li[64] = {0, 5, 6, 7, 12, 13, 14, 18, 19, 20, 21, 25, 26, 27, 32, 33, 34, 39, 40, 41, 44, 45, 46, 47, 51, 52, 53, 54, 58, 59, 60, 61, 1, 2, 3, 4, 8, 9, 10, 11, 15, 16, 17, 22, 23, 24, 28, 29, 30, 31, 35, 36, 37, 38, 42, 43, 48, 49, 50, 55, 56, 57, 62, -1};
+    static const int32_t dw[63] = {42, 34, 25, 14, 21, 7, 18, 0, 36, 38, 62, 60, 41, 63, 48, 3, 33, 61, 40, 55, 47, 28, 39, 31, 32, 17, 51, 26, 52, 23, 24, 10, 2, 1, 29, 8, 44, 15, 37, 22, 6, 11, 50, 58, 4, 12, 30, 59, 54, 19, 43, 20, 45, 46, 5, 49, 13, 16, 35, 57, 56, 53, 9};
+    for (int x = 0; x < 64; ++x) lane_item[x] = li[x];
+    for (int x = 0; x < 63; ++x) dword[x] = dw[x];
+    return true;
+  }
+  default: return false;
+  }
+}
+
 // the shape has a register-C2V float32 decoder
 inline bool float_shape_ok(int shape) {
   switch (shape) {

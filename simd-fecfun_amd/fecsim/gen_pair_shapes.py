@@ -123,6 +123,26 @@ def emit(csrc_dir: str) -> None:
             h.append("      if (cols[e] != c[e] || shifts[e] != s[e]) return false;")
             h.append("    return true;")
             h.append("  }")
+    # LDS layouts of k_pair_i8c (bank conflicts), searched by lds_layout.py
+    from . import lds_layout
+    lay = lds_layout.load()
+    h += ["  default: return false;", "  }", "}", "",
+          "// k_pair_i8c's LDS layout of the shape (lds_layout.py): per hardware lane the",
+          "// (slot, lr) item slot * Z/2 + lr (-1: spare lane), per (slot, position) item its",
+          "// dword in a column row; false: slots in lane order, positions in slot order",
+          "inline bool pair_shape_lds_layout(int shape, int32_t* lane_item, int32_t* dword) {",
+          "  switch (shape) {"]
+    for i, (w, ms, (zc, mc)) in enumerate(sh):
+        if zc and ms in lay:
+            li, dw = lay[ms]["lane_item"], lay[ms]["dword"]
+            h.append("  case %d: {  // %s: %.3f LDS cycles per access (%.3f in lane order)" % (
+                i, ms, lay[ms]["cycles_per_access"], lay[ms]["identity_cycles_per_access"]))
+            h.append("    static const int32_t li[64] = {%s};" % ", ".join(map(str, li)))
+            h.append("    static const int32_t dw[%d] = {%s};" % (len(dw), ", ".join(map(str, dw))))
+            h.append("    for (int x = 0; x < 64; ++x) lane_item[x] = li[x];")
+            h.append("    for (int x = 0; x < %d; ++x) dword[x] = dw[x];" % len(dw))
+            h.append("    return true;")
+            h.append("  }")
     fl = float_shapes()
     h += ["  default: return false;", "  }", "}", "",
           "// the shape has a register-C2V float32 decoder",
