@@ -235,6 +235,7 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
     if (he == hipSuccess)
       he = hipMemcpy(d.d_pair_adr, d.pair.adrtab.data(), sizeof(uint32_t) * d.pair.adrtab.size(),
                      hipMemcpyHostToDevice);
+    if (he == hipSuccess && d.pair.ct) he = hipMalloc(&d.d_pair_queue, 64 * 256);
     if (he != hipSuccess) {
       fecsim_decoder_free(dec);
       return hip_fail(he, "pair kernel tables to device");
@@ -332,6 +333,7 @@ void fecsim_decoder_free(fecsim_decoder* d) {
   if (d->d.d_edges) (void)hipFree(d->d.d_edges);
   if (d->d.d_layer_w) (void)hipFree(d->d.d_layer_w);
   if (d->d.d_pair_adr) (void)hipFree(d->d.d_pair_adr);
+  if (d->d.d_pair_queue) (void)hipFree(d->d.d_pair_queue);
   if (d->d.d_c2v_scratch) (void)hipFree(d->d.d_c2v_scratch);
   if (d->d.d_beta_tab) (void)hipFree(d->d.d_beta_tab);
   if (d->d.d_beta_tab8) (void)hipFree(d->d.d_beta_tab8);

@@ -198,6 +198,8 @@ struct PairLayout {
   int half = 0, cw_per_wave = 0, col_dwords = 0, wave_dwords = 0, waves = 0;
   bool ct = false;                // the edge table is compiled into the shape's k_pair_i8c
   size_t ct_off = 0;              // k_pair_i8c's per-lane table in adrtab (when ct)
+  int lead[4] = {};               // k_pair_i8c: lane of each slot's row 0 and each slot's lanes
+  unsigned long long smask[4] = {};
   std::vector<uint32_t> adrtab;   // [half][total_hc_ones], then k_pair_i8c's table
 };
 
@@ -212,6 +214,7 @@ struct DecoderDev {
   int freg_shape = -1;            // compiled code shape for the register-C2V float kernel, -1 if none
   PairLayout pair;
   uint32_t* d_pair_adr = nullptr;
+  unsigned* d_pair_queue = nullptr;  // k_pair_i8c's 64 work-queue counters (one per 256 bytes)
   int kernel = FECSIM_KERNEL_AUTO;
   void* d_c2v_scratch = nullptr;  // generic kernel, c2v_global shapes only
   uint32_t max_grid = 0;          // blocks the generic kernel is launched with at most (0: no cap)

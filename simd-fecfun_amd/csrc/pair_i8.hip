@@ -93,6 +93,10 @@ struct PairArgs {
   int8_t* c2v;             // [num_cw][E] or null
   unsigned long long* tot_iter;
   int ct;                  // the decoder's edge table is the shape's compiled-in one (k_pair_i8c)
+  unsigned* ctr;           // k_pair_i8c work queue: counter g of codeword range g at ctr[64 g]
+  int qgroups;             // number of ranges (and counters), at most 64
+  int lead[4];             // k_pair_i8c: lane of each slot's row 0, and each slot's lanes
+  unsigned long long smask[4];
 };
 
 // waves per SIMD the register allocation aims at: 4 when the edge addresses, the packed
@@ -401,9 +405,43 @@ k_pair_i8(const PairArgs a) {
 //   per half, Sa the sign mask of the row's XOR: (m << 8) * 0xffff = -(m << 8) mod 2^16.
 // * mv2 = min(cap, sat(m2 - beta)), which equals min(max(m2, beta), cap + beta) - beta for
 //   |Q| <= 128.
+// Per edge: its column is written by no later layer of the iteration, so its posterior
+// after the edge's own layer is the one the syndrome reads (taken from the register, not
+// read back); per layer: it has such an edge.
+template <class SH>
+struct PairFinal {
+  struct Tab {
+    bool e[SH::ne];
+    bool l[SH::mrow];
+  };
+  static constexpr Tab make() {
+    Tab t{};
+    int e0 = 0;
+    for (int l = 0; l < SH::mrow; ++l) {
+      t.l[l] = false;
+      for (int j = 0; j < SH::w[l]; ++j) {
+        bool fin = true;
+        int x = e0 + SH::w[l];
+        for (int l2 = l + 1; l2 < SH::mrow; ++l2) {
+          for (int k = 0; k < SH::w[l2]; ++k)
+            if (SH::ecol[x + k] == SH::ecol[e0 + j]) fin = false;
+          x += SH::w[l2];
+        }
+        t.e[e0 + j] = fin;
+        t.l[l] = t.l[l] || fin;
+      }
+      e0 += SH::w[l];
+    }
+    return t;
+  }
+  static constexpr Tab tab = make();
+};
+
+// 4 waves per SIMD when the C2V registers, the class addresses and a layer's working set
+// fit 128 VGPRs (802.11ad 1/2), else no constraint (the other shapes take 129-147)
 template <class SH>
 constexpr int pair_c_waves_per_eu() {
-  return SH::ne + 2 * SH::ncls + 3 * SH::maxw + 20 <= 128 ? 4 : 1;
+  return SH::ne + 2 * SH::ncls + 3 * SH::maxw + 14 < 128 ? 4 : 1;
 }
 
 template <class SH, bool BT>
@@ -415,10 +453,10 @@ k_pair_i8c(const PairArgs a) {
   const int lane = threadIdx.x;              // one wave per block
   // per-lane layout from pair_layout (a.adrtab): the class addresses (absolute LDS byte
   // addresses, the wave's columns start at 0), the lane's pair position in a column row,
-  // its row lr, the lane of its slot's row 0 and the lanes of its slot.  Lanes, slots and
-  // pair positions are placed to keep LDS bank conflicts low; the spare lane mirrors a
-  // lane of its own half-wave (same addresses, same values: a broadcast)
-  const uint32_t* lt = a.adrtab + (size_t)lane * (NC + 4);
+  // its row lr and slot (a.lead / a.smask give each slot's row-0 lane and lanes).  Lanes,
+  // slots and pair positions are placed to keep LDS bank conflicts low; the spare lane
+  // mirrors a lane of its own half-wave (same addresses, same values: a broadcast)
+  const uint32_t* lt = a.adrtab + (size_t)lane * (NC + 2);
   uint32_t ga[NC], gb[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
@@ -428,21 +466,31 @@ k_pair_i8c(const PairArgs a) {
   const int pos = (int)lt[NC];
   const uint32_t info = lt[NC + 1];
   const int lr = (int)(info & 0xffu);
-  const int lead = (int)((info >> 8) & 0xffu);
+  const int sl = (int)((info >> 8) & 0xffu);   // slot
   const bool real = ((info >> 16) & 1u) != 0u;
-  const unsigned long long smask = (unsigned long long)lt[NC + 2] | ((unsigned long long)lt[NC + 3] << 32);
   uint32_t* wbase = smem;
 
-  const long long w0 = (long long)blockIdx.x * a.chunk;
-  long long w1 = w0 + a.chunk;
-  if (w1 > (long long)a.num_cw) w1 = a.num_cw;
-  long long next = w0;
-  long long cw = -1;
+  // work queue: the grid is the resident blocks, and the blocks of group g = blockIdx %
+  // qgroups take batches of QB codewords of the group's share of the range from counter g;
+  // the next batch is fetched one ahead.  Every block works until its group's range is
+  // done, so no block idles while another still holds a long static chunk; one counter
+  // per 64 blocks keeps the same-address atomics far below their ~100 ns serial rate.
+  constexpr int QB = 8;
+  const int qg = (int)(blockIdx.x % (unsigned)a.qgroups);
+  const long long gs = (long long)a.num_cw * qg / a.qgroups, ge = (long long)a.num_cw * (qg + 1) / a.qgroups;
+  unsigned vq = 0;
+  if (lane == 0) vq = atomicAdd(a.ctr + qg * 64, (unsigned)QB);
+  long long bcur = gs + (long long)__builtin_amdgcn_readfirstlane(vq);
+  long long bend = bcur + QB < ge ? bcur + QB : ge;
+  bool exhausted = bcur >= ge;
+  if (!exhausted && lane == 0) vq = atomicAdd(a.ctr + qg * 64, (unsigned)QB);   // next batch, in flight
+  uint32_t cw = 0;                           // codeword of this lane's slot (when active)
   bool active = false;
   uint32_t iter = 0;
-  uint32_t r[NE];                            // C2V of edge e: rows lr, lr + L as int16 halves v << 8
+  typedef uint32_t __attribute__((ext_vector_type(2))) u32x2;
+  u32x2 rp[(NE + 1) / 2];                    // C2V of edges 2k, 2k+1: rows lr, lr + L as int16 halves v << 8
 #pragma unroll
-  for (int e = 0; e < NE; ++e) r[e] = 0u;
+  for (int k = 0; k < (NE + 1) / 2; ++k) rp[k] = u32x2{0u, 0u};
   const ushort2v beta2 = ushort2v{(unsigned short)(a.beta << 8), (unsigned short)(a.beta << 8)};
   const ushort2v cap2 = ushort2v{(unsigned short)(a.cap << 8), (unsigned short)(a.cap << 8)};
   unsigned long long my_iters = 0;
@@ -456,18 +504,39 @@ k_pair_i8c(const PairArgs a) {
 
   for (;;) {
     // ---- refill: idle slots take the next codewords of the block's range ----
+    int lead = a.lead[0];
+#pragma unroll
+    for (int k = 1; k < 64 / L; ++k) lead = sl == k ? a.lead[k] : lead;
     const unsigned long long want = __ballot(real && lane == lead && !active);
-    if (want != 0ull && next < w1) {
+    if (want != 0ull && !exhausted) {
+      const int nwant = __popcll(want);
       const int rank = __popcll(want & ((1ull << lead) - 1ull));
-      const long long mine = next + rank;
-      const bool take = !active && mine < w1;
-      next += __popcll(want);
+      const long long avail = bend - bcur;
+      long long mine = bcur + rank;
+      bool valid = rank < avail;
+      if (nwant > avail) {   // the batch runs out: continue with the prefetched one
+        const long long bn = gs + (long long)__builtin_amdgcn_readfirstlane(vq);
+        const long long be = bn + QB < ge ? bn + QB : ge;
+        if (!valid) {
+          mine = bn + (rank - avail);
+          valid = mine < be;
+        }
+        bcur = bn + (nwant - avail);
+        bend = be;
+        exhausted = bn >= ge;
+        if (!exhausted && lane == 0) vq = atomicAdd(a.ctr + qg * 64, (unsigned)QB);
+      } else {
+        bcur += nwant;
+      }
+      const bool take = !active && valid;
       if (take) {
-        cw = mine;
+        cw = (uint32_t)mine;
         active = true;
         iter = 0;
         if (real) {
-          const uint8_t* g = reinterpret_cast<const uint8_t*>(a.llr_in) + cw * (long long)(SH::mc * ZC) + lr;
+          int lro = lr;
+          asm volatile("" : "+v"(lro));   // per-lane pointers formed here, not kept across the loop
+          const uint8_t* g = reinterpret_cast<const uint8_t*>(a.llr_in) + (long long)cw * (SH::mc * ZC) + lro;
           uint32_t* wp = wbase + pos;
           static_assert(SH::mc % 4 == 0, "column groups of four");
 #pragma unroll
@@ -487,9 +556,10 @@ k_pair_i8c(const PairArgs a) {
           }
         }
       }
-      const uint32_t keep = take ? 0u : ~0u;   // fresh codeword: C2V messages start at zero
+      if (take) {   // fresh codeword: C2V messages start at zero (64-bit moves on its lanes)
 #pragma unroll
-      for (int e = 0; e < NE; ++e) r[e] &= keep;
+        for (int k = 0; k < (NE + 1) / 2; ++k) asm volatile("v_mov_b64 %0, 0" : "=v"(rp[k]));
+      }
     }
     if (__ballot(active) == 0ull) break;
     wave_sync();
@@ -498,6 +568,7 @@ k_pair_i8c(const PairArgs a) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) asm volatile("" : "+v"(ga[k]), "+v"(gb[k]));   // edge addresses formed at each use
     int e0 = 0;
+    uint32_t psyn[SH::mrow];   // per layer, XOR of the posteriors of its final edges
     const uint8_t* bt_it = btab + (iter < a.max_iter ? iter : a.max_iter - 1u) * SH::mrow;
 #pragma unroll
     for (int l = 0; l < SH::mrow; ++l) {
@@ -520,7 +591,7 @@ k_pair_i8c(const PairArgs a) {
           const int e = e0 + j;
           const uint32_t ad = (SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + SH::ecol[e] * 512;
           const uint32_t x = *lds32(ad);
-          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(r[e]));
+          const short2v qs = __builtin_elementwise_sub_sat(as_s(x), as_s(rp[e >> 1][e & 1]));
           const short2v qq = as_s(as_u(qs) & 0xff00ff00u);   // Q << 8, Q = sat8(L - R)
           q[j] = qq;
           const ushort2v av = BT ? as_us(as_u(smax(qq, -qq)) & amask) : as_us(as_u(smax(qq, -qq)));
@@ -547,9 +618,13 @@ k_pair_i8c(const PairArgs a) {
           const uint32_t k = __builtin_amdgcn_bitop3_b32(sa, qa, 0x00010001u, 0xBE);   // (sa ^ qa) | 1
           const uint32_t rn = as_u(as_us(as_u(m)) * as_us(k));
           gu[j] = as_u(__builtin_elementwise_add_sat(q[j], as_s(rn)));   // L = sat8(Q + R)
-          r[e0 + j] = rn;
+          rp[(e0 + j) >> 1][(e0 + j) & 1] = rn;
         }
       }
+      psyn[l] = 0u;
+#pragma unroll
+      for (int j = 0; j < MAXW; ++j)
+        if (j < w && PairFinal<SH>::tab.e[e0 + j]) psyn[l] ^= gu[j];
 #pragma unroll
       for (int j = 0; j < MAXW; ++j) {
         if (j < w) {
@@ -574,29 +649,36 @@ k_pair_i8c(const PairArgs a) {
     for (int l = 0; l < SH::mrow; ++l) {
       const int w = SH::w[l];
       uint32_t xs[SH::maxw];
+      int n = 0;
 #pragma unroll
       for (int j = 0; j < SH::maxw; ++j)
-        if (j < w) {
+        if (j < w && !PairFinal<SH>::tab.e[e0 + j]) {
           const int e = e0 + j;
-          xs[j] = *lds32((SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + SH::ecol[e] * 512);
+          xs[n++] = *lds32((SH::eflip[e] ? gb[SH::ecls[e]] : ga[SH::ecls[e]]) + SH::ecol[e] * 512);
         }
+      if (PairFinal<SH>::tab.l[l]) xs[n++] = psyn[l];
       uint32_t par = xs[0];
 #pragma unroll
       for (int j = 1; j + 1 < SH::maxw; j += 2)
-        if (j + 1 < w) par = xor3(par, xs[j], xs[j + 1]);
-      if (w & 1) F |= par;
-      else F = __builtin_amdgcn_bitop3_b32(F, par, xs[w - 1], 0xF6);   // F | (par ^ x)
+        if (j + 1 < n) par = xor3(par, xs[j], xs[j + 1]);
+      if (n & 1) F |= par;
+      else F = __builtin_amdgcn_bitop3_b32(F, par, xs[n - 1], 0xF6);   // F | (par ^ x)
       e0 += w;
       __builtin_amdgcn_sched_barrier(0);
     }
     const bool fail = (F & 0x80008000u) != 0u;
     const unsigned long long fb = __ballot(real && active && fail);
     ++iter;
+    unsigned long long smask = a.smask[0];
+#pragma unroll
+    for (int k = 1; k < 64 / L; ++k) smask = sl == k ? a.smask[k] : smask;
     const bool cwfail = (fb & smask) != 0ull;
     const bool finish = active && (!cwfail || iter == a.max_iter);   // LDPCdecode.cpp:192-197
     if (finish && real) {
       const uint32_t* wp = wbase + pos;
-      uint8_t* g = reinterpret_cast<uint8_t*>(a.post_out) + cw * (long long)(SH::mc * ZC) + lr;
+      int lro = lr;
+      asm volatile("" : "+v"(lro));
+      uint8_t* g = reinterpret_cast<uint8_t*>(a.post_out) + (long long)cw * (SH::mc * ZC) + lro;
       static_assert(SH::mc % 8 == 0, "column groups of eight");
 #pragma unroll
       for (int c0 = 0; c0 < SH::mc; c0 += 8) {
@@ -614,11 +696,11 @@ k_pair_i8c(const PairArgs a) {
       if (a.c2v) {
         int zo = ZC;
         asm volatile("" : "+v"(zo));
-        int8_t* gc = a.c2v + cw * (long long)(NE * ZC) + lr;
+        int8_t* gc = a.c2v + (long long)cw * (NE * ZC) + lro;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-          gc[e * zo] = (int8_t)(r[e] >> 8);
-          gc[e * zo + L] = (int8_t)(r[e] >> 24);
+          gc[e * zo] = (int8_t)(rp[e >> 1][e & 1] >> 8);
+          gc[e * zo + L] = (int8_t)(rp[e >> 1][e & 1] >> 24);
         }
       }
       if (lr == 0) {
@@ -654,6 +736,11 @@ int pair_i8_shape(const DecoderDev& d) {
   const int c8 = (int)(int8_t)(int)mrm;
   if (b8 < 1 || c8 < 0) return -1;
   return pair_shape_index(h.mrow, h.row_weights.data());
+}
+
+// clears k_pair_i8c's 64 queue counters (one per 256 bytes)
+__global__ void k_pair_queue_reset(unsigned* c) {
+  c[threadIdx.x * 64] = 0u;
 }
 
 // CT: the instance with the code's edge table compiled in (k_pair_i8c)
@@ -698,8 +785,19 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   }
   const long long slots_per_block = (long long)a.waves * a.cw_per_wave;
   long long blocks = (a.num_cw + slots_per_block - 1) / slots_per_block;
-  const long long cap_blocks = a.max_blocks > 0 ? a.max_blocks : 8LL * resident;
-  if (blocks > cap_blocks) blocks = cap_blocks;
+  if constexpr (CT) {
+    // persistent grid of the resident blocks; the counters (the decoder's, so calls on one
+    // decoder must not overlap) are cleared by a kernel, not a memset, so that the pair can
+    // be captured in a graph
+    if (!a.ctr) return hipErrorInvalidValue;
+    if (blocks > resident) blocks = resident;
+    a.qgroups = blocks < 64 ? (int)blocks : 64;
+    hipLaunchKernelGGL(k_pair_queue_reset, dim3(1), dim3(64), 0, st, a.ctr);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+  } else {
+    const long long cap_blocks = a.max_blocks > 0 ? a.max_blocks : 8LL * resident;
+    if (blocks > cap_blocks) blocks = cap_blocks;
+  }
   const long long waves = blocks * a.waves;
   a.chunk = (a.num_cw + waves - 1) / waves;
   hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(64 * a.waves), lds, st, a);
@@ -750,8 +848,8 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
   if (!pl->ct) return;
   // k_pair_i8c's per-lane table, behind the run-time table: per hardware lane, the
   // addresses of the shift classes (distinct s mod Z/2 in order of first use, as the shape
-  // header lists them) in row [c + lr >= Z/2], the pair position, lr | lead lane << 8 |
-  // real << 16, and the 64-bit mask of the slot's lanes
+  // header lists them) in row [c + lr >= Z/2], the pair position and lr | slot << 8 |
+  // real << 16; per slot its row-0 lane and the 64-bit mask of its lanes (kernel arguments)
   std::vector<int> cls;
   for (int e = 0; e < NE; ++e) {
     const int c = h.row_offset[e] % L;
@@ -788,7 +886,7 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
       if (item_of[x] % L == 0) lead_of_slot[sl] = x;
       mask_of_slot[sl] |= 1ull << x;
     }
-  const size_t stride = (size_t)nc + 4;
+  const size_t stride = (size_t)nc + 2;
   pl->adrtab.resize(pl->ct_off + 64 * stride);
   uint32_t* t = pl->adrtab.data() + pl->ct_off;
   for (int x = 0; x < 64; ++x) {
@@ -798,9 +896,11 @@ void pair_layout(const HdecHost& h, PairLayout* pl) {
       t[x * stride + k] = (uint32_t)(o * 256 + 4 * dword[sl * L + t0 - o * L]);
     }
     t[x * stride + nc] = (uint32_t)dword[sl * L + lr];
-    t[x * stride + nc + 1] = (uint32_t)lr | ((uint32_t)lead_of_slot[sl] << 8) | ((real[x] ? 1u : 0u) << 16);
-    t[x * stride + nc + 2] = (uint32_t)mask_of_slot[sl];
-    t[x * stride + nc + 3] = (uint32_t)(mask_of_slot[sl] >> 32);
+    t[x * stride + nc + 1] = (uint32_t)lr | ((uint32_t)sl << 8) | ((real[x] ? 1u : 0u) << 16);
+  }
+  for (int sl = 0; sl < 4; ++sl) {
+    pl->lead[sl] = sl < cpw ? lead_of_slot[sl] : 0;
+    pl->smask[sl] = sl < cpw ? mask_of_slot[sl] : 0ull;
   }
 }
 
@@ -809,6 +909,11 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   const PairLayout& pl = d.pair;
   PairArgs a{};
   a.ct = pl.ct && !run_time_table ? 1 : 0;
+  a.ctr = d.d_pair_queue;
+  for (int sl = 0; sl < 4; ++sl) {
+    a.lead[sl] = pl.lead[sl];
+    a.smask[sl] = pl.smask[sl];
+  }
   a.adrtab = d.d_pair_adr + (a.ct ? pl.ct_off : 0);
   a.z = h.z;
   a.half = pl.half;
