@@ -538,17 +538,22 @@ k_pair_i8c(const PairArgs a) {
           asm volatile("" : "+v"(lro));   // per-lane pointers formed here, not kept across the loop
           const uint8_t* g = reinterpret_cast<const uint8_t*>(a.llr_in) + (long long)cw * (SH::mc * ZC) + lro;
           uint32_t* wp = wbase + pos;
-          static_assert(SH::mc % 4 == 0, "column groups of four");
+          // columns in two groups of mc / 2: the byte loads of a group in flight together (two
+          // memory round trips per refill; all 2 * mc at once would spill beside the C2V and
+          // address registers), then per column the pair (lo, hi) and its twin (hi, lo) as
+          // int16 halves v << 8
+          static_assert(SH::mc % 2 == 0, "two column groups");
+          constexpr int CG = SH::mc / 2;
 #pragma unroll
-          for (int c0 = 0; c0 < SH::mc; c0 += 4) {
-            uint32_t lo[4], hi[4];
+          for (int c0 = 0; c0 < SH::mc; c0 += CG) {
+            uint32_t lo[CG], hi[CG];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < CG; ++k) {
               lo[k] = g[(c0 + k) * ZC];
               hi[k] = g[(c0 + k) * ZC + L];
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < CG; ++k) {
               wp[(c0 + k) * 128] = __builtin_amdgcn_perm(hi[k], lo[k], 0x040c000cu);
               wp[(c0 + k) * 128 + 64] = __builtin_amdgcn_perm(lo[k], hi[k], 0x040c000cu);
             }
