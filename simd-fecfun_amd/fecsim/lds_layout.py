@@ -7,8 +7,9 @@ and lr + L of every layer; for an edge of shift s it reads the pair at position
 away, in the same bank).  Per iteration every edge is read twice (layer and syndrome)
 and written twice, all with the same bank pattern.
 
-Bank model (measured on MI355X: the LDS-active and bank-conflict cycles of the
-run-time-table kernel match it to 2%): a wave's 32-bit LDS access is served in two
+Bank model (on MI355X the run-time-table kernel's LDS-active cycles per LDS instruction,
+3.93 with its refill and write-out included, are within 3% of the model's 3.81 for its
+decode accesses): a wave's 32-bit LDS access is served in two
 passes of 32 lanes (lanes 0-31, then 32-63) over 32 banks of 4 bytes; a pass takes as
 many cycles as the most distinct dwords one bank is asked for (equal addresses are a
 broadcast).  With the slots in lane order and the pair positions in slot order (the
