@@ -38,9 +38,9 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz, 32-bi
 VALU_PK16_PEAK_TOPS = 2 * VALU_PEAK_TOPS
 OPS_PER_EDGE_UPDATE = 13                    # algorithmic int ops per edge-update (offset min-sum)
 E_11AD = 2184                               # edges per iteration, 11ad 1/2
-# HBM bytes per codeword moved by one k_pair_i8 launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# runs scaled by a calibration copy (profiles/k_pair_i8_11ad12_1e6cw_hbm_traffic.md)
-PAIR_TRAFFIC_BYTES_PER_CW = 1358.0
+# HBM bytes per codeword moved by one k_pair_i8c launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs scaled by a calibration copy (profiles/k_pair_i8c_11ad12_1e6cw_hbm_traffic.md)
+PAIR_TRAFFIC_BYTES_PER_CW = 1370.6
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12     # 256 CUs x 128 B/clk x 2.4 GHz
 LDS_BYTES_PER_EDGE_UPDATE = 8
 
