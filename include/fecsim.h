@@ -142,7 +142,10 @@ void fecsim_decoder_free(fecsim_decoder *d);
  *   c2v      device [num_cw][total_hc_ones*z] final check-to-variable messages (row r
  *            of edge e at e*z + r, rows labeled as the .dat's row_offset), or NULL
  *   tot_iter device uint64 to which the summed iterations are added, or NULL
- *            (the return value of LDPCdecode_set) */
+ *            (the return value of LDPCdecode_set)
+ * At most 8 decodes of one decoder may be in flight on the device at once (on
+ * different streams): the row-pair kernel's work queue takes one of the decoder's 8
+ * counter sets per call, in turn.  A captured graph keeps the set of its capture. */
 int fecsim_decode_set(fecsim_decoder *d, uint32_t num_cw, int layout_p, void *metrics, uint32_t *iters,
                       void *c2v, unsigned long long *tot_iter, void *stream);
 

@@ -235,7 +235,7 @@ int fecsim_decoder_create(const fecsim_hdec* hd, const char* kind, const fecsim_
     if (he == hipSuccess)
       he = hipMemcpy(d.d_pair_adr, d.pair.adrtab.data(), sizeof(uint32_t) * d.pair.adrtab.size(),
                      hipMemcpyHostToDevice);
-    if (he == hipSuccess && d.pair.ct) he = hipMalloc(&d.d_pair_queue, 64 * 256);
+    if (he == hipSuccess && d.pair.ct) he = hipMalloc(&d.d_pair_queue, kPairQueueSets * kPairQueueSetBytes);
     if (he != hipSuccess) {
       fecsim_decoder_free(dec);
       return hip_fail(he, "pair kernel tables to device");

@@ -193,6 +193,10 @@ struct MultiArgs {
   int tab_words, tab_bytes;
 };
 
+// k_pair_i8c's counter sets per decoder: that many of its decodes may overlap on the device
+constexpr int kPairQueueSets = 8;
+constexpr size_t kPairQueueSetBytes = 64 * 256;
+
 // Layout of the row-pair int8 kernel (pair_i8.hip)
 struct PairLayout {
   int half = 0, cw_per_wave = 0, col_dwords = 0, wave_dwords = 0, waves = 0;
@@ -214,7 +218,9 @@ struct DecoderDev {
   int freg_shape = -1;            // compiled code shape for the register-C2V float kernel, -1 if none
   PairLayout pair;
   uint32_t* d_pair_adr = nullptr;
-  unsigned* d_pair_queue = nullptr;  // k_pair_i8c's 64 work-queue counters (one per 256 bytes)
+  unsigned* d_pair_queue = nullptr;  // k_pair_i8c's work-queue counters: kPairQueueSets sets of 64
+                                     // (one per 256 bytes), one set per launch in rotation
+  mutable unsigned pair_queue_next = 0;
   int kernel = FECSIM_KERNEL_AUTO;
   void* d_c2v_scratch = nullptr;  // generic kernel, c2v_global shapes only
   uint32_t max_grid = 0;          // blocks the generic kernel is launched with at most (0: no cap)

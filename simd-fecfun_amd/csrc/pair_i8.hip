@@ -791,9 +791,9 @@ static hipError_t launch_shape_z(const PairArgs& in, hipStream_t st) {
   const long long slots_per_block = (long long)a.waves * a.cw_per_wave;
   long long blocks = (a.num_cw + slots_per_block - 1) / slots_per_block;
   if constexpr (CT) {
-    // persistent grid of the resident blocks; the counters (the decoder's, so calls on one
-    // decoder must not overlap) are cleared by a kernel, not a memset, so that the pair can
-    // be captured in a graph
+    // persistent grid of the resident blocks; the counters (one of the decoder's sets, used
+    // in rotation) are cleared by a kernel, not a memset, so that the pair can be captured
+    // in a graph
     if (!a.ctr) return hipErrorInvalidValue;
     if (blocks > resident) blocks = resident;
     a.qgroups = blocks < 64 ? (int)blocks : 64;
@@ -914,7 +914,10 @@ hipError_t launch_pair_i8(const DecoderDev& d, const KernelArgs& ka, hipStream_t
   const PairLayout& pl = d.pair;
   PairArgs a{};
   a.ct = pl.ct && !run_time_table ? 1 : 0;
-  a.ctr = d.d_pair_queue;
+  if (a.ct) {   // the next counter set: up to kPairQueueSets decodes of one decoder may overlap
+    const unsigned k = __atomic_fetch_add(&d.pair_queue_next, 1u, __ATOMIC_RELAXED) % kPairQueueSets;
+    a.ctr = d.d_pair_queue + (size_t)k * (kPairQueueSetBytes / sizeof(unsigned));
+  }
   for (int sl = 0; sl < 4; ++sl) {
     a.lead[sl] = pl.lead[sl];
     a.smask[sl] = pl.smask[sl];

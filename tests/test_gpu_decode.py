@@ -243,6 +243,40 @@ def test_gpu_pair_kernel_matches_oracle(case, cuda):
         assert int(tot.item()) == ref[3]
 
 
+def test_gpu_pair_kernel_concurrent_streams(cuda):
+    """Twelve decodes of one 802.11ad 1/2 decoder issued on two streams without waiting
+    in between (the work-queue kernel takes one of the decoder's 8 counter sets per call,
+    in turn): every batch equals the oracle."""
+    import torch
+    from fecsim import lib as L
+    cs = codes.gen_structs("11ad 1/2", simd_size=512, elem_size=8)
+    rs = refsim.RefStream(cs)
+    batches = []
+    for snr, ncw in ((-1.0, 3000), (-0.5, 2000), (0.0, 1000)):
+        wgn, bits = rs.inner_loop(ncw)
+        llr, _ = rs.metrics(wgn, bits, snr)
+        batches.append((llr, O.decode_ideal(cs.dec, llr)))
+    d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
+    assert d.kernel == L.KERNEL_PAIR
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    ins = [torch.from_numpy(b[0]).cuda() for b in batches]
+    torch.cuda.synchronize()
+    outs, its = [], []
+    for k in range(12):
+        b = k % len(batches)
+        st = streams[k % 2]
+        with torch.cuda.stream(st):   # outputs made (and zeroed) on the stream that decodes
+            out = torch.empty_like(ins[b])
+            it = torch.zeros(ins[b].shape[0], dtype=torch.int32, device="cuda")
+            d.decode(ins[b], out, iters=it, stream=st)
+        outs.append((b, out, it))
+    torch.cuda.synchronize()
+    for b, out, it in outs:
+        ref = batches[b][1]
+        assert np.array_equal(out.cpu().numpy(), ref[0]), b
+        assert np.array_equal(it.cpu().numpy().astype(np.uint32), ref[2]), b
+
+
 @pytest.mark.parametrize("ms", ["11ad 1/2", "11n 1/2 z54", "11ad 13/16"])
 def test_gpu_pair_kernel_extreme_rows(ms, cuda):
     """Rows whose Q values are mostly -128 with one 0 (|Q| = 128 beside |Q| = 0, the
