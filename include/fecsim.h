@@ -413,10 +413,16 @@ void fecsim_msim_free(fecsim_msim *s);
  * (iteration, layer): C2V_j = sign * alpha * min(cap, min_{k != j} max(|q_k| - beta, 0)).
  * With alpha = 1 and beta = beta_offset it is the Vec16f_min / Vec8d_min layer update
  * (src/LDPCDec/decode_functions.cpp:11-103).  The forward keeps checkpoints
- * (posteriors and C2V) only at the starts of ceil(T/K) segments, K = ceil(sqrt(T)),
- * the last segment K iterations long; the adjoint recomputes each earlier segment from
- * its checkpoint, keeping that segment's V2C messages, and sweeps it backwards.  Device
- * memory per codeword: ceil(T/K) (N + E) + (K + 2) E values, plus 2 N when the
+ * (posteriors and C2V) only at the starts of ceil(T/K) segments, the last segment K
+ * iterations long, and the per-iteration state the adjoint needs for that last segment;
+ * the adjoint recomputes each earlier segment from its checkpoint, keeping that
+ * segment's state, and sweeps it backwards.  The state of an iteration is S values: its
+ * V2C messages (S = E), or, for the kernels that hold the C2V messages in registers
+ * (float32 or FECSIM_I8, Z <= 128, the compiled rate-1/2 shapes of 802.11n, 802.11ad and
+ * 802.16e), each row's two smallest magnitudes and a word of indices and per-edge flags
+ * (S = 3 mrow z, 4 mrow z with FECSIM_I8).  K is the segment length in 1..T of least
+ * memory ceil(T/K) (N + E) + K S (the larger on a tie), of order sqrt(T (N + E) / S).
+ * Device memory per codeword: ceil(T/K) (N + E) + K S + 2 E values, plus 2 N when the
  * posteriors sit in global memory (N = mcol z variables, E = total_hc_ones z edges). */
 typedef struct fecsim_trainer fecsim_trainer;
 
@@ -430,7 +436,14 @@ typedef struct {
                             [-128, 127], offsets rounded to integers (the gradient passes the
                             rounding unchanged, a saturated value passes none), an offset of 0
                             giving A = 0 at -128 as abs(-128) does; alpha should be 1 */
+    uint32_t flags;      /* 0, or FECSIM_TRAIN_GLOBAL_C2V */
 } fecsim_train_params;
+
+/* fecsim_train_params.flags: keep the C2V messages in a global array even where the code's
+ * shape has kernels that hold them in registers (float, Z <= 128, the compiled rate-1/2
+ * shapes of 802.11n, 802.11ad and 802.16e).  Both give the same results bit for bit; the
+ * register kernels are faster. */
+#define FECSIM_TRAIN_GLOBAL_C2V 1u
 
 int fecsim_trainer_create(const fecsim_hdec *h, const fecsim_train_params *p, fecsim_trainer **out);
 /* llr, post: device [batch][mcol*z]; alpha, beta: device [T][mrow]; all of elem's type

@@ -20,7 +20,8 @@ LIB = os.path.join(OUT_DIR, "libfecsim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["decode.hip", "float_reg.hip", "pair_i8.hip", "code_i8.hip", "sim.hip", "refrng.hip", "train.hip", "multi.hip", "api.cpp", "sim_api.cpp", "msim_api.cpp",
+SOURCES = ["decode.hip", "float_reg.hip", "pair_i8.hip", "code_i8.hip", "sim.hip", "refrng.hip", "train.hip", "train_reg.hip",
+           "multi.hip", "api.cpp", "sim_api.cpp", "msim_api.cpp",
            "train_api.cpp", "multi_api.cpp"]
 
 FLAGS = [

@@ -151,17 +151,20 @@ struct TrainArgs {
   int waves;                 // codewords (waves) per block
   int is_double, global_p;   // element type; posteriors in global scratch instead of LDS
   int quant;                 // float32 emulating the int8 decoder (saturation, integer offsets)
+  int reg_shape;             // compiled shape of the register-C2V kernels (train_reg.hip), -1 if none
   double cap;
   uint64_t batch;
   const void* llr;           // [batch][n]
   const void* alpha;         // [T][mrow]
   const void* beta;          // [T][mrow]
   void* post;                // [batch][n]
-  int seg;                   // K: checkpoint every K iterations (ceil(sqrt(T)))
+  int seg;                   // K: checkpoint every K iterations (the K of least adjoint memory)
   void* ckP;                 // [batch][ceil(T/K)][n] posteriors at the start of iterations 0, K, 2K, ...
   void* ckC;                 // [batch][ceil(T/K)][e] C2V at those iterations
   void* wc;                  // [batch][e] working C2V
-  void* qs;                  // [batch][K][e] V2C messages of the segment being reversed
+  void* qs;                  // [batch][K][S] kept state of the segment being reversed: the V2C
+                             // messages (S = e), or the register kernels' reverse state
+                             // (S = mrow * train_reg_state_words * z)
   void* gc;                  // [batch][e] adjoint of the C2V messages
   void* partial;             // [batch][T][mrow][2] per-codeword parameter gradients
   void* gscratch;            // [batch][2][n] posteriors + adjoints when global_p
@@ -267,6 +270,12 @@ int shape_with_rows(const HdecHost& h, int elem_bytes, int rows_per_lane, DecSha
 hipError_t launch_train(const TrainArgs& a, int bwd, hipStream_t stream);
 // waves of that kernel (with a's global_p and waves) resident per CU (0: query failed)
 int train_resident_waves(const TrainArgs& a, int bwd);
+// register-C2V kernels of the differentiable decoder (train_reg.hip): the compiled shape
+// for code h (float32 elements, Z <= 128), or -1; and their launch (forward or adjoint)
+int train_reg_shape(const HdecHost& h, int is_double);
+hipError_t launch_train_reg(int shape, const TrainArgs& a, int bwd, hipStream_t stream);
+// words a row and layer of the register kernels' kept reverse state (quant: int8 emulation)
+int train_reg_state_words(int quant);
 int encode_shape(int z, int mcol, EncArgs* a);
 hipError_t launch_encode(const EncArgs& a, hipStream_t stream);
 hipError_t launch_llr(int elem, const LlrArgs& a, hipStream_t stream);

@@ -8,17 +8,18 @@
 // With alpha = 1 and a constant beta it equals the reference decoder on codewords
 // that need all T iterations.
 //
-// Memory: the forward keeps checkpoints only at the starts of NS = ceil(T / K) segments,
-// K = ceil(sqrt(T)) (posteriors P and C2V C), and one working C2V array.  The segments
-// are [0, f), [f, f + K), ..., [T - K, T) with f = T - (NS - 1) K: the last one, whose
-// V2C messages the forward keeps, is a full K long, so the adjoint runs T - K
-// iterations again (not T - f).  The adjoint takes the segments in reverse: it restarts
+// Memory: the forward keeps checkpoints only at the starts of NS = ceil(T / K) segments
+// (posteriors P and C2V C; K is chosen by the trainer to minimise the memory, of order
+// sqrt(T)), and one working C2V array.  The segments are [0, f), [f, f + K), ...,
+// [T - K, T) with f = T - (NS - 1) K: the last one, whose V2C messages the forward keeps,
+// is a full K long, so the adjoint runs T - K iterations again (not T - f).  The adjoint takes the segments in reverse: it restarts
 // from the segment's checkpoint and runs the segment's iterations again, keeping their
 // V2C messages q (the only values the reverse sweep needs), then sweeps those iterations
 // backwards, layer by layer:
 //   g_out_j = gP[v_j] + gC[e_j]           (C2V_j feeds the posterior and iteration t+1)
 //   g_m_j   = sign_j * g_out_j,  g_alpha += g_m_j * min(cap, x_j),  g_A[src_j] += alpha * g_m_j (x_j < cap)
 //   g_q_j   = gP[v_j] + [|q_j| > beta] * sign(q_j) * g_A_j,  g_beta -= [|q_j| > beta] * g_A_j
+//             (g_A_j is nonzero only at the argmin and the second minimum)
 //   gP[v_j] <- g_q_j,  gC[e_j] <- -g_q_j    (gradient w.r.t. the C2V entering iteration t)
 // where x_j is the exclusive minimum and src_j the edge it came from: the row's argmin
 // for every other edge, its second minimum for the argmin itself.  Per codeword and
@@ -34,16 +35,16 @@
 
 #include "decode_core.h"
 #include "internal.h"
+#include "train_core.h"
 
 namespace fecsim {
 
 namespace {
 
+using namespace train_core;
+
 constexpr int kTrainMaxW = 32;
 
-// IEEE sign bit as 0/1 (-0.0 counts as negative, like the reference's sign_combine)
-__device__ __forceinline__ int sbit(float x) { return (int)(__float_as_uint(x) >> 31); }
-__device__ __forceinline__ int sbit(double x) { return (int)((unsigned long long)__double_as_longlong(x) >> 63); }
 
 template <typename F>
 struct TrainView {
@@ -64,19 +65,6 @@ __device__ __forceinline__ TrainView<F> train_view(const TrainArgs& a, uint64_t 
   return v;
 }
 
-// int8 decoder emulation (QT): saturation of V2C and posteriors to int8, and the
-// magnitude of an offset-0 row: abs(-128) = -128 gives A = max(-128, 0) = 0
-// (decode_functions.cpp:48-61,88-90).  On integer-valued inputs with integer offsets
-// every value stays an integer, exact in float.
-template <typename F>
-__device__ __forceinline__ F sat8f(F x) { return x < F(-128) ? F(-128) : (x > F(127) ? F(127) : x); }
-template <typename F, bool QT>
-__device__ __forceinline__ F mag_a(F q, F be) {
-  const F x = fabs(q) - be;
-  const F A = x > F(0) ? x : F(0);
-  if constexpr (QT) return (be == F(0) && q == F(-128)) ? F(0) : A;
-  return A;
-}
 
 // one check row of the forward: C_old -> C_new, P updated in place; optionally keeps q
 // (with QT: the V2C value before its saturation). MAXW bounds the row weight w (the
@@ -120,6 +108,13 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
 // parameter gradients to *ga, *gb
 // with QT a saturated value passes no gradient (the clamps), and the offset's rounding
 // passes it straight through (d round(b) / db taken as 1)
+//
+// Summation order (train_reg.hip follows it operation for operation): with gm_j the C2V
+// adjoint times the C2V sign, S = sum over the edges other than the argmin in edge order,
+// the scale's gradient gets c1 S then c2 gm_argmin, A_argmin's gradient is alpha S, the
+// second minimum's alpha gm_argmin; only the argmin and the second-minimum edge take a
+// term back (each if its magnitude was positive), and the offset's gradient subtracts
+// those two in edge order.
 template <typename F, int MAXW, bool QT>
 __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* gP, F* gC, const F* Qs, int e0, int w,
                                         int r, int Z, F al, F be, F cap, F* ga, F* gb) {
@@ -141,67 +136,47 @@ __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* g
       else if (A < m2) { m2 = A; im2 = j; }
     }
   }
-  // gradient reaching A_k: every edge other than the argmin took its value from the
-  // argmin (summed in edge order), the argmin took it from the second minimum
-  F g_im = F(0), g_im2 = F(0), gaa = *ga, gbb = *gb;
+  // every edge other than the argmin took its magnitude from the argmin, the argmin from
+  // the second minimum
+  const bool cp1 = cap < m1, cp2 = cap < m2;
+  const F c1 = cp1 ? cap : m1, c2 = cp2 ? cap : m2;
+  F S = F(0), gmi = F(0);
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
       const int ce = (e0 + j) * Z + r;
-      const F x = (j == im) ? m2 : m1;
-      const bool capped = cap < x;
-      const F mc = capped ? cap : x;
       const F s = (sgn ^ sbit(q[j])) ? F(-1) : F(1);
       F gv = gP[vi[j]];
       if constexpr (QT) {   // the posterior's saturation
-        const F y = q[j] + s * (al * mc);
+        const F y = q[j] + s * (al * ((j == im) ? c2 : c1));
         if (y < F(-128) || y > F(127)) gv = F(0);
       }
       const F gm = s * (gv + gC[ce]);
       gq[j] = gv;
-      gaa += gm * mc;
-      if (!capped) {
-        if (j == im) g_im2 += al * gm;
-        else g_im += al * gm;
-      }
+      if (j == im) gmi = gm;
+      else S += gm;
     }
   }
+  *ga = (*ga + c1 * S) + c2 * gmi;
+  const F g_im = cp1 ? F(0) : al * S, g_im2 = cp2 ? F(0) : al * gmi;
+  const bool act1 = im >= 0 && m1 > F(0), act2 = im2 >= 0 && m2 > F(0);   // |q| - beta > 0 there
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
-      const F gA = (j == im) ? g_im : ((j == im2) ? g_im2 : F(0));
-      const bool wrap = QT && be == F(0) && q[j] == F(-128);   // A = 0 there, constant
-      if (fabs(q[j]) - be > F(0) && !wrap) {
-        gq[j] += sbit(q[j]) ? -gA : gA;
-        gbb -= gA;
-      }
-      const F g = qin[j] ? gq[j] : F(0);
+      F g = gq[j];
+      if (j == im && act1) g += sbit(q[j]) ? -g_im : g_im;
+      if (j == im2 && act2) g += sbit(q[j]) ? -g_im2 : g_im2;
+      g = qin[j] ? g : F(0);
       gP[vi[j]] = g;
       gC[(e0 + j) * Z + r] = -g;
     }
   }
-  *ga = gaa;
-  *gb = gbb;
+  const F a1 = act1 ? g_im : F(0), a2 = act2 ? g_im2 : F(0);
+  *gb = (im < im2) ? (*gb - a1) - a2 : (*gb - a2) - a1;
 }
 
-template <typename F>
-__device__ __forceinline__ F wave_sum(F v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
-// QT offsets: rounded to integers (the int8 decoder's table takes integers)
-template <typename F, bool QT>
-__device__ __forceinline__ F offset_of(F b) {
-  if constexpr (QT) return rint(b);
-  return b;
-}
 
-// first iteration of segment s: [0, f), [f, f + K), ... with f = T - (NS - 1) K
-__device__ __forceinline__ int seg_start(int s, int T, int K, int NS) {
-  return s == 0 ? 0 : T - (NS - s) * K;
-}
 
 template <typename F, bool GP, int MAXW, bool QT>
 __global__ void __launch_bounds__(256) k_train_fwd(const TrainArgs a) {
@@ -311,8 +286,14 @@ __global__ void __launch_bounds__(256) k_train_bwd(const TrainArgs a) {
         const int w = layer_w[l];
         e0 -= w;
         const F al = alpha[t * Lr + l], be = offset_of<F, QT>(beta[t * Lr + l]);
+        // per lane: the sum of its rows' sums, each row's terms summed from 0 in edge order
         F ga = F(0), gb = F(0);
-        for (int r = lane; r < Z; r += 64) rev_row<F, MAXW, QT>(edges, gP, gC, Qs, e0, w, r, Z, al, be, cap, &ga, &gb);
+        for (int r = lane; r < Z; r += 64) {
+          F pa = F(0), pb = F(0);
+          rev_row<F, MAXW, QT>(edges, gP, gC, Qs, e0, w, r, Z, al, be, cap, &pa, &pb);
+          ga = r == lane ? pa : ga + pa;
+          gb = r == lane ? pb : gb + pb;
+        }
         wave_sync();
         ga = wave_sum(ga);
         gb = wave_sum(gb);
@@ -364,12 +345,14 @@ hipError_t launch_tw(const TrainArgs& a, int bwd, hipStream_t st) {
 }
 
 // the kernels' row loops unroll to the smallest weight bound that holds the code's rows
+// (the register-C2V kernels of train_reg.hip when the code's shape is compiled for them)
 template <typename F, bool GP>
 hipError_t launch_t(const TrainArgs& a, int bwd, hipStream_t st) {
-  hipError_t e = a.maxw <= 8    ? launch_tw<F, GP, 8>(a, bwd, st)
-                 : a.maxw <= 16 ? launch_tw<F, GP, 16>(a, bwd, st)
-                 : a.maxw <= 24 ? launch_tw<F, GP, 24>(a, bwd, st)
-                                : launch_tw<F, GP, kTrainMaxW>(a, bwd, st);
+  hipError_t e = std::is_same<F, float>::value && a.reg_shape >= 0 ? launch_train_reg(a.reg_shape, a, bwd, st)
+                 : a.maxw <= 8                                      ? launch_tw<F, GP, 8>(a, bwd, st)
+                 : a.maxw <= 16                                     ? launch_tw<F, GP, 16>(a, bwd, st)
+                 : a.maxw <= 24                                     ? launch_tw<F, GP, 24>(a, bwd, st)
+                                                                    : launch_tw<F, GP, kTrainMaxW>(a, bwd, st);
   if (e != hipSuccess || !bwd) return e;
   const int per_cw = a.iters * a.mrow * 2;
   hipLaunchKernelGGL(k_train_sum<F>, dim3(per_cw), dim3(256), 0, st, reinterpret_cast<const F*>(a.partial),

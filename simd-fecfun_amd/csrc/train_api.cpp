@@ -1,6 +1,7 @@
 // C ABI of the differentiable decoder (declared in include/fecsim.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <new>
 #include <string>
 #include <vector>
@@ -15,6 +16,7 @@ struct fecsim_trainer {
   int elem_bytes = 4;
   int n_pad = 0, tab_bytes = 0;
   int waves[2] = {4, 4}, global_p[2] = {0, 0};   // per kernel: [forward, backward]
+  int reg_shape = -1;   // compiled shape of the register-C2V kernels, -1: train.hip's kernels
   uint32_t* d_edges = nullptr;
   int32_t* d_layer_w = nullptr;
   int seg = 1;   // checkpoint every seg iterations
@@ -68,6 +70,7 @@ TrainArgs base_args(const fecsim_trainer* t, uint32_t batch, int bwd) {
   a.waves = t->waves[bwd];
   a.is_double = t->p.elem == FECSIM_F64;
   a.quant = t->p.elem == FECSIM_I8;
+  a.reg_shape = t->reg_shape;
   a.global_p = t->global_p[bwd];
   a.cap = (double)t->p.cap;
   a.batch = batch;
@@ -91,6 +94,7 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (p->elem != FECSIM_F32 && p->elem != FECSIM_F64 && p->elem != FECSIM_I8)
     return fail(FECSIM_E_INVAL, "elem must be FECSIM_F32, FECSIM_F64 or FECSIM_I8");
   if (p->iters == 0 || p->iters > 1000 || p->max_batch == 0) return fail(FECSIM_E_INVAL, "need 1..1000 iterations and max_batch >= 1");
+  if (p->flags & ~FECSIM_TRAIN_GLOBAL_C2V) return fail(FECSIM_E_INVAL, "unknown trainer flags");
   const HdecHost& h = hd->h;
   if (h.max_row_weight > kTrainMaxW) return fail(FECSIM_E_UNSUPPORTED, "row weight above 32");
   fecsim_trainer* t = new (std::nothrow) fecsim_trainer();
@@ -98,6 +102,9 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   t->h = h;
   t->p = *p;
   t->elem_bytes = p->elem == FECSIM_F64 ? 8 : 4;
+  // C2V messages in registers (one wave per codeword, posteriors in LDS) when the code's
+  // shape is compiled for it: no global C2V traffic on the layers' serial chain
+  t->reg_shape = (p->flags & FECSIM_TRAIN_GLOBAL_C2V) ? -1 : train_reg_shape(h, p->elem == FECSIM_F64);
   const size_t N = (size_t)h.mcol * h.z, E = (size_t)h.total_hc_ones * h.z, T = p->iters, B = p->max_batch;
   t->n_pad = (int)((N + 3) / 4 * 4);
   t->tab_bytes = ((h.total_hc_ones + h.mrow) * 4 + 15) / 16 * 16;   // edge words + layer weights in LDS
@@ -128,16 +135,26 @@ int fecsim_trainer_create(const fecsim_hdec* hd, const fecsim_train_params* p, f
   if (he == hipSuccess) he = hipMalloc((void**)&t->d_layer_w, sizeof(int32_t) * h.mrow);
   if (he == hipSuccess) he = hipMemcpy(t->d_edges, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(t->d_layer_w, h.row_weights.data(), sizeof(int32_t) * h.mrow, hipMemcpyHostToDevice);
-  // checkpoints at the starts of ceil(T / K) segments, K = ceil(sqrt(T)), the last one K
-  // long; the adjoint recomputes one segment of at most K iterations at a time
-  size_t K = 1;
-  while (K * K < T) ++K;
+  // checkpoints (posteriors and C2V, N + E values) at the starts of ceil(T / K) segments, the
+  // last one K long; the adjoint recomputes one segment of at most K iterations at a time,
+  // keeping S values per iteration (the V2C messages, S = E, or the register kernels'
+  // reverse state).  K is the segment length of least memory ceil(T / K) (N + E) + K S
+  // (the larger K on a tie), of order sqrt(T (N + E) / S)
+  const size_t S = t->reg_shape >= 0 ? (size_t)h.mrow * train_reg_state_words(p->elem == FECSIM_I8) * h.z : E;
+  size_t K = 1, best = SIZE_MAX;
+  for (size_t k = 1; k <= T; ++k) {
+    const size_t m = (T + k - 1) / k * (N + E) + k * S;
+    if (m <= best) {
+      best = m;
+      K = k;
+    }
+  }
   t->seg = (int)K;
   const size_t NS = (T + K - 1) / K;
   if (he == hipSuccess) he = hipMalloc(&t->ckP, B * NS * N * eb);
   if (he == hipSuccess) he = hipMalloc(&t->ckC, B * NS * E * eb);
-  if (he == hipSuccess) he = hipMalloc(&t->wc, B * E * eb);
-  if (he == hipSuccess) he = hipMalloc(&t->qs, B * K * E * eb);
+  if (he == hipSuccess && t->reg_shape < 0) he = hipMalloc(&t->wc, B * E * eb);   // working C2V of train.hip
+  if (he == hipSuccess) he = hipMalloc(&t->qs, B * K * S * eb);
   if (he == hipSuccess) he = hipMalloc(&t->gc, B * E * eb);
   if (he == hipSuccess) he = hipMalloc(&t->partial, B * T * h.mrow * 2 * eb);
   for (int bwd = 0; bwd < 2 && he == hipSuccess; ++bwd) {

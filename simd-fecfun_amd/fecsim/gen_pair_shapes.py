@@ -1,7 +1,8 @@
-"""Generates csrc/pair_shapes.h, csrc/pair_shapes_dispatch.h and
-csrc/float_shapes_dispatch.h: the code shapes (layer count and row weights) the
-row-pair int8 decoder (csrc/pair_i8.hip) and the register-C2V float decoder
-(csrc/float_reg.hip) are compiled for.
+"""Generates csrc/pair_shapes.h, csrc/pair_shapes_dispatch.h,
+csrc/float_shapes_dispatch.h, csrc/train_shapes.h and csrc/train_shapes_dispatch.h: the
+code shapes (layer count and row weights) the row-pair int8 decoder (csrc/pair_i8.hip),
+the register-C2V float decoder (csrc/float_reg.hip) and the register-C2V differentiable
+decoder (csrc/train_reg.hip) are compiled for.
 
 A shape is the row-weight sequence of a base matrix; the shifts and columns stay run-time
 data, so one shape serves every lifting size, num_par and row relabeling of a code.
@@ -29,6 +30,20 @@ MAX_EDGES = 90
 # shapes the float32 decoder with C2V in registers is compiled for (the float workloads:
 # rate-1/2 codes with row weights <= 8; one C2V register per edge of a row's layers)
 FLOAT_SETS = ["11ad 1/2", "11n 1/2 z54", "11n 1/2 z27", "16e 1/2"]
+# shapes the differentiable decoder keeps its C2V messages in registers for (csrc/train_reg.hip:
+# rate-1/2 codes, Z <= 128; two rows per lane of 86 edges at most: 172 registers)
+TRAIN_SETS = ["11n 1/2 z81", "11n 1/2 z54", "11n 1/2 z27", "11ad 1/2", "16e 1/2"]
+
+
+def train_shapes():
+    """[(row weights, matrix set)] of the register-C2V trainer, one per distinct shape"""
+    out = []
+    for ms in TRAIN_SETS:
+        hc, _, _ = codes.get_hmatrix(ms)
+        w = tuple(int((r >= 0).sum()) for r in hc)
+        if w not in [x for x, _ in out]:
+            out.append((w, ms))
+    return out
 
 
 def float_shapes():
@@ -158,6 +173,29 @@ def emit(csrc_dir: str) -> None:
     d += ["  default: return hipErrorInvalidValue;", "  }", "}", ""]
     _write(os.path.join(csrc_dir, "pair_shapes.h"), "\n".join(h))
     _write(os.path.join(csrc_dir, "pair_shapes_dispatch.h"), "\n".join(d))
+    ts = train_shapes()
+    t = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.", "#pragma once", "#include <stdint.h>", "",
+         "namespace fecsim {", "",
+         "// code shapes (layer count and row weights) of the register-C2V differentiable decoder",
+         "// (csrc/train_reg.hip); shifts and columns stay run-time data"]
+    for i, (w, ms) in enumerate(ts):
+        t.append("struct TrainShape%d {  // %s" % (i, ms))
+        t.append("  static constexpr int mrow = %d, ne = %d, maxw = %d;" % (len(w), sum(w), max(w)))
+        t.append("  static constexpr int w[%d] = {%s};" % (len(w), ", ".join(map(str, w))))
+        t.append("};")
+    t += ["", "inline int train_shape_index(int mrow, const int32_t* w) {"]
+    for i, (w, _) in enumerate(ts):
+        cond = " && ".join("w[%d] == %d" % (k, x) for k, x in enumerate(w))
+        t.append("  if (mrow == %d && %s) return %d;" % (len(w), cond, i))
+    t += ["  return -1;", "}", "", "}  // namespace fecsim", ""]
+    _write(os.path.join(csrc_dir, "train_shapes.h"), "\n".join(t))
+    td = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
+          "// included inside namespace fecsim by train_reg.hip after FECSIM_TRAIN_LAUNCH is defined",
+          "hipError_t train_reg_dispatch(int shape, const TrainArgs& a, int bwd, hipStream_t st) {", "  switch (shape) {"]
+    for i in range(len(ts)):
+        td.append("  case %d: FECSIM_TRAIN_LAUNCH(TrainShape%d)" % (i, i))
+    td += ["  default: return hipErrorInvalidValue;", "  }", "}", ""]
+    _write(os.path.join(csrc_dir, "train_shapes_dispatch.h"), "\n".join(td))
     f = ["// Generated by fecsim/gen_pair_shapes.py. Do not edit.",
          "// included inside namespace fecsim by float_reg.hip after FECSIM_FREG_LAUNCH is defined",
          "template <class P>", "hipError_t float_dispatch(int shape, const KernelArgs& a, hipStream_t st) {",

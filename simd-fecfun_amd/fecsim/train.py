@@ -18,7 +18,11 @@ from .decoder import Decoder, Hdec, _ptr
 
 
 class TrainParams(C.Structure):
-    _fields_ = [("iters", C.c_uint32), ("max_batch", C.c_uint32), ("cap", C.c_float), ("elem", C.c_int)]
+    _fields_ = [("iters", C.c_uint32), ("max_batch", C.c_uint32), ("cap", C.c_float), ("elem", C.c_int),
+                ("flags", C.c_uint32)]
+
+
+GLOBAL_C2V = 1   # keep the C2V messages in a global array (the kernels for any shape)
 
 
 def _stream():
@@ -30,13 +34,14 @@ class Trainer:
     """Device state of the unrolled decoder: checkpoints for up to max_batch codewords.
     int8=True: float32 emulation of the int8 decoder (integer-valued LLRs in the
     decoder's units, saturation to [-128, 127], offsets rounded with a straight-through
-    gradient)."""
+    gradient).  global_c2v=True keeps the C2V messages in a global array even where the
+    code's shape has the register kernels (same results bit for bit, slower)."""
 
     def __init__(self, hdec: Hdec, iters: int, max_batch: int, cap: float = 1e30, double: bool = False,
-                 int8: bool = False):
+                 int8: bool = False, global_c2v: bool = False):
         self.hdec, self.iters, self.max_batch, self.cap = hdec, iters, max_batch, cap
         self.elem = _L.I8 if int8 else (_L.F64 if double else _L.F32)
-        p = TrainParams(iters, max_batch, cap, self.elem)
+        p = TrainParams(iters, max_batch, cap, self.elem, GLOBAL_C2V if global_c2v else 0)
         t = C.c_void_p()
         _L.check(_L.lib().fecsim_trainer_create(hdec._h, C.byref(p), C.byref(t)), "fecsim_trainer_create")
         self._t = t

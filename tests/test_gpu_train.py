@@ -335,3 +335,37 @@ def test_gpu_train_backward_keeps_its_parameters(cuda):
     assert np.array_equal(gb.cpu().numpy(), gb2.cpu().numpy())
     with pytest.raises(FecsimError):
         tr.backward(w)
+
+
+@pytest.mark.parametrize("name,T,cap,int8", [("11n 1/2 z81", 10, 6.0, False), ("11n 1/2 z54", 7, 1e30, False),
+                                            ("11n 1/2 z27", 4, 1e30, False), ("11ad 1/2", 9, 1e30, False),
+                                            ("16e 1/2", 5, 8.0, False), ("11ad 1/2", 6, 31.0, True),
+                                            ("11n 1/2 z81", 5, 31.0, True)])
+def test_gpu_train_register_kernels_match_global_c2v(name, T, cap, int8, cuda):
+    """The register-C2V kernels (compiled shapes, one wave per codeword) against the
+    global-C2V kernels forced by FECSIM_TRAIN_GLOBAL_C2V: posteriors and all three
+    gradients equal bit for bit, over several checkpoint segments (T >= 5 recomputes)
+    and a batch that is not a multiple of the global kernels' waves per block."""
+    import torch
+    from fecsim.train import Trainer
+    cs = _cs(name)
+    B = 45
+    if int8:
+        x = np.clip(_llrs(cs, B, 0.0, O.I8, 2.0, 24.0).astype(np.float32) * 3, -128, 127)
+    else:
+        x = _llrs(cs, B, -0.5).astype(np.float32)
+    rng = np.random.default_rng(31)
+    al = np.ones((T, cs.dec.mrow), np.float32) if int8 else rng.uniform(0.6, 1.0, (T, cs.dec.mrow)).astype(np.float32)
+    be = rng.uniform(0.0, 3.0 if int8 else 0.9, (T, cs.dec.mrow)).astype(np.float32)
+    w = rng.normal(size=x.shape).astype(np.float32)
+    t = lambda a: torch.from_numpy(a).cuda()   # noqa: E731
+    res = []
+    for glob in (False, True):
+        tr = Trainer(Hdec.from_tables(cs.dec), T, 64, cap=cap, int8=int8, global_c2v=glob)
+        post = tr.forward(t(x), t(al), t(be))
+        gl, ga, gb = tr.backward(t(w))
+        torch.cuda.synchronize()
+        res.append([v.cpu().numpy() for v in (post, gl, ga, gb)])
+    for a, b, what in zip(res[0], res[1], ("post", "grad_llr", "grad_alpha", "grad_beta")):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), what
+    assert np.abs(res[0][2]).max() > 0 and np.abs(res[0][3]).max() > 0

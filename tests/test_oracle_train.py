@@ -96,3 +96,24 @@ def test_train_gradient_matches_torch_autograd(oracle):
     np.testing.assert_allclose(ga, ta.grad.numpy(), rtol=1e-10, atol=1e-10)
     np.testing.assert_allclose(gb, tb.grad.numpy(), rtol=1e-10, atol=1e-10)
     np.testing.assert_allclose(gl, tx.grad.numpy(), rtol=1e-10, atol=1e-10)
+
+
+def test_register_trainer_shapes_cover_their_sets():
+    """Every matrix set named for the register-C2V trainer has its decoder's layer weights
+    (in gen_structs' layer order, which the trainer's tables follow) among the compiled
+    shapes of the generated header, each shape once."""
+    import os
+    from fecsim import gen_pair_shapes
+    from fecsim.codes import gen_structs
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(here, "simd-fecfun_amd", "csrc", "train_shapes.h")).read()
+    shapes = gen_pair_shapes.train_shapes()
+    assert len({w for w, _ in shapes}) == len(shapes)
+    for ms in gen_pair_shapes.TRAIN_SETS:
+        d = gen_structs(ms, simd_size=512, elem_size=8).dec
+        w = tuple(int(x) for x in d.row_weights)
+        i = [x for x, _ in shapes].index(w)
+        assert "static constexpr int mrow = %d, ne = %d, maxw = %d;" % (len(w), sum(w), max(w)) in hdr
+        cond = " && ".join("w[%d] == %d" % (k, x) for k, x in enumerate(w))
+        assert "if (mrow == %d && %s) return %d;" % (len(w), cond, i) in hdr
+        assert d.z <= 128   # one or two rows per lane
