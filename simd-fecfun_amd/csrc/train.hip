@@ -104,17 +104,17 @@ __device__ __forceinline__ void fwd_row(const TrainArgs& a, const uint32_t* __re
   }
 }
 
-// one check row of the reverse sweep (see the head of this file); adds the row's
-// parameter gradients to *ga, *gb
+// one check row of the reverse sweep (see the head of this file); sets the row's
+// parameter-gradient terms *ga, *gb
 // with QT a saturated value passes no gradient (the clamps), and the offset's rounding
 // passes it straight through (d round(b) / db taken as 1)
 //
 // Summation order (train_reg.hip follows it operation for operation): with gm_j the C2V
 // adjoint times the C2V sign, S = sum over the edges other than the argmin in edge order,
-// the scale's gradient gets c1 S then c2 gm_argmin, A_argmin's gradient is alpha S, the
-// second minimum's alpha gm_argmin; only the argmin and the second-minimum edge take a
-// term back (each if its magnitude was positive), and the offset's gradient subtracts
-// those two in edge order.
+// the scale's term is c1 S + c2 gm_argmin, A_argmin's gradient is alpha S, the second
+// minimum's alpha gm_argmin; only the argmin and the second-minimum edge take a term back
+// (each if its magnitude was positive), and the offset's term is minus the sum of those
+// two.
 template <typename F, int MAXW, bool QT>
 __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* gP, F* gC, const F* Qs, int e0, int w,
                                         int r, int Z, F al, F be, F cap, F* ga, F* gb) {
@@ -157,9 +157,10 @@ __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* g
       else S += gm;
     }
   }
-  *ga = (*ga + c1 * S) + c2 * gmi;
+  *ga = c1 * S + c2 * gmi;
   const F g_im = cp1 ? F(0) : al * S, g_im2 = cp2 ? F(0) : al * gmi;
-  const bool act1 = im >= 0 && m1 > F(0), act2 = im2 >= 0 && m2 > F(0);   // |q| - beta > 0 there
+  // |q| - beta > 0 there (without an argmin every magnitude is infinite, and capped)
+  const bool act1 = m1 > F(0), act2 = m2 > F(0);
 #pragma unroll
   for (int j = 0; j < MAXW; ++j) {
     if (j < w) {
@@ -171,8 +172,7 @@ __device__ __forceinline__ void rev_row(const uint32_t* __restrict__ edges, F* g
       gC[(e0 + j) * Z + r] = -g;
     }
   }
-  const F a1 = act1 ? g_im : F(0), a2 = act2 ? g_im2 : F(0);
-  *gb = (im < im2) ? (*gb - a1) - a2 : (*gb - a2) - a1;
+  *gb = -((act1 ? g_im : F(0)) + (act2 ? g_im2 : F(0)));
 }
 
 

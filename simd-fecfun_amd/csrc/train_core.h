@@ -23,24 +23,23 @@ __device__ __forceinline__ F mag_a(F q, F be) {
   return A;
 }
 
-// sum over the 64 lanes of a wave (all lanes active), the same value in every lane.  float:
-// four DPP steps within each row of 16 lanes (partner lane ^ 1, ^ 2, the mirrored quad, the
-// mirrored half-row), then the four row sums read from lanes 0, 16, 32, 48 and added as
-// (r0 + r16) + (r32 + r48): a fixed order, so every kernel that sums with it rounds alike
-template <int CTRL>
+// sum over the 64 lanes of a wave (all lanes active), wave-uniform.  float: four DPP steps
+// within each row of 16 lanes (partner lane ^ 1, ^ 2, the mirrored quad, the mirrored
+// half-row) leave row sums R0..R3 in their rows; two row broadcasts then form
+// (R3 + R2) + (R1 + R0) in lane 63: a fixed order, so every kernel that sums with it rounds
+// alike
+template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_plus(float v) {
-  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
 __device__ __forceinline__ float wave_sum(float v) {
-  v = dpp_plus<0xB1>(v);    // quad_perm [1, 0, 3, 2]
-  v = dpp_plus<0x4E>(v);    // quad_perm [2, 3, 0, 1]
-  v = dpp_plus<0x141>(v);   // row_half_mirror
-  v = dpp_plus<0x140>(v);   // row_mirror
-  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-  const float r16 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-  const float r32 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-  const float r48 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-  return (r0 + r16) + (r32 + r48);
+  v = dpp_plus<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
+  v = dpp_plus<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
+  v = dpp_plus<0x141, 0xF>(v);   // row_half_mirror
+  v = dpp_plus<0x140, 0xF>(v);   // row_mirror
+  v = dpp_plus<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3: R1 + R0, R3 + R2
+  v = dpp_plus<0x143, 0x8>(v);   // row_bcast:31 into row 3: (R3 + R2) + (R1 + R0)
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

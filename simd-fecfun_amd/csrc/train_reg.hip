@@ -58,12 +58,16 @@ constexpr int layer_start(int L) {
 template <bool QT>
 constexpr int kStateWords = QT ? 4 : 3;
 
-// packed bits: [0, W) V2C sign bits (edge j at bit W - 1 - j), [16, 20) index of m1,
-// [20, 24) index of m2 (15: none), [24, 32) V2C within int8 (int8 emulation, edge j at
-// bit 24 + j); word 3: posterior saturated (edge j at bit j).  Whether the two minima's
-// offsets were active (|q| - beta > 0) is m1 > 0 and m2 > 0.
-__device__ __forceinline__ uint32_t pack_state(uint32_t qm, int im, int im2, uint32_t inm) {
-  return qm | ((uint32_t)(im & 15) << 16) | ((uint32_t)(im2 & 15) << 20) | (inm << 24);
+// packed bits: [0, W) the C2V sign of each edge (edge j at bit W - 1 - j: the row's sign
+// parity and the V2C sign), [12] the V2C sign of the m1 edge, [13] of the m2 edge,
+// [16, 20) index of m1, [20, 24) index of m2 (15: none), [24, 32) V2C within int8 (int8
+// emulation, edge j at bit 24 + j); word 3: posterior saturated (edge j at bit j).  Whether
+// the two minima's offsets were active (|q| - beta > 0) is m1 > 0 and m2 > 0.
+template <int W>
+__device__ __forceinline__ uint32_t pack_state(uint32_t qm, uint32_t sgb, int im, int im2, uint32_t inm) {
+  const uint32_t fl = (qm ^ (uint32_t)((int32_t)sgb >> 31)) & ((1u << W) - 1u);   // qm: V2C signs, sgb: parity
+  const uint32_t q1 = (qm >> (W - 1 - im)) & 1u, q2 = (qm >> (W - 1 - im2)) & 1u;   // im = -1: shift by W, 0
+  return fl | (q1 << 12) | (q2 << 13) | ((uint32_t)(im & 15) << 16) | ((uint32_t)(im2 & 15) << 20) | (inm << 24);
 }
 
 constexpr uint32_t kSign = 0x80000000u;
@@ -159,7 +163,7 @@ __device__ __forceinline__ void fwd_row_reg(const uint32_t (&tw)[W], float* c, f
   if constexpr (KEEP) {
     St[r] = m1;
     St[Z + r] = m2;
-    reinterpret_cast<uint32_t*>(St)[2 * Z + r] = pack_state(qm, im, im2, inm);
+    reinterpret_cast<uint32_t*>(St)[2 * Z + r] = pack_state<W>(qm, sgb, im, im2, inm);
     if constexpr (QT) reinterpret_cast<uint32_t*>(St)[3 * Z + r] = ym;
   }
 }
@@ -167,7 +171,8 @@ __device__ __forceinline__ void fwd_row_reg(const uint32_t (&tw)[W], float* c, f
 // one check row of the reverse sweep with its W C2V adjoints in registers, held negated
 // (ng[j] = -gC_j, so gv + gC_j is gv - ng[j]), its edges' table words (adjoint columns)
 // in tw and its reverse state in st: train.hip's rev_row operation for operation, its
-// conditionals as selects and its multiplications by +-1 as sign changes
+// conditionals as selects and its multiplications by +-1 as sign changes; sets the row's
+// parameter-gradient terms *ga, *gb
 template <int W, bool QT>
 __device__ __forceinline__ void rev_row_reg(const uint32_t (&tw)[W], float* ng, const uint32_t (&st)[kStateWords<QT>],
                                             uint32_t r4, uint32_t rz4, float al, float cap, float* ga, float* gb) {
@@ -180,9 +185,6 @@ __device__ __forceinline__ void rev_row_reg(const uint32_t (&tw)[W], float* ng, 
   const float m1 = __uint_as_float(st[0]), m2 = __uint_as_float(st[1]);
   const uint32_t b = st[2];
   const int im = (int)((b >> 16) & 15u), im2 = (int)((b >> 20) & 15u);
-  const uint32_t qm = b & 0xffu;
-  // bit W - 1 - j: the sign of edge j's C2V (the row's sign parity and the V2C sign)
-  const uint32_t fl = qm ^ (0u - (uint32_t)(__popc(qm) & 1));
   const bool cp1 = cap < m1, cp2 = cap < m2;
   const float c1 = cp1 ? cap : m1, c2 = cp2 ? cap : m2;
   float S = 0.f, gmi = 0.f;
@@ -190,27 +192,25 @@ __device__ __forceinline__ void rev_row_reg(const uint32_t (&tw)[W], float* ng, 
   for (int j = 0; j < W; ++j) {
     const bool isim = j == im;
     if constexpr (QT) gq[j] = ((st[3] >> j) & 1u) ? 0.f : gq[j];
-    const float gm = __uint_as_float(__float_as_uint(gq[j] - ng[j]) ^ ((fl << (32 - W + j)) & kSign));
+    const float gm = __uint_as_float(__float_as_uint(gq[j] - ng[j]) ^ ((b << (32 - W + j)) & kSign));
     const float Sn = S + gm;
     S = isim ? S : Sn;
     gmi = isim ? gm : gmi;
   }
-  *ga = (*ga + c1 * S) + c2 * gmi;
+  *ga = c1 * S + c2 * gmi;
   const float g_im = cp1 ? 0.f : al * S, g_im2 = cp2 ? 0.f : al * gmi;
-  const bool act1 = im != 15 && m1 > 0.f, act2 = im2 != 15 && m2 > 0.f;
+  const bool act1 = m1 > 0.f, act2 = m2 > 0.f;
   // the terms the two minima's edges take back (-0 adds nothing)
-  const float T1 = act1 ? __uint_as_float(__float_as_uint(g_im) ^ (((qm >> ((W - 1 - im) & 31)) & 1u) << 31)) : -0.f;
-  const float T2 = act2 ? __uint_as_float(__float_as_uint(g_im2) ^ (((qm >> ((W - 1 - im2) & 31)) & 1u) << 31)) : -0.f;
+  const float T1 = act1 ? __uint_as_float(__float_as_uint(g_im) ^ ((b << 19) & kSign)) : -0.f;
+  const float T2 = act2 ? __uint_as_float(__float_as_uint(g_im2) ^ ((b << 18) & kSign)) : -0.f;
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    const float x1 = gq[j] + T1, x2 = gq[j] + T2;
-    float g = (j == im) ? x1 : ((j == im2) ? x2 : gq[j]);
+    float g = gq[j] + ((j == im) ? T1 : ((j == im2) ? T2 : -0.f));   // + -0 leaves gq as it is
     if constexpr (QT) g = ((b >> (24 + j)) & 1u) ? g : 0.f;
     lds_st(ad[j], g);
     ng[j] = g;
   }
-  const float a1 = act1 ? g_im : 0.f, a2 = act2 ? g_im2 : 0.f;
-  *gb = (im < im2) ? (*gb - a1) - a2 : (*gb - a2) - a1;
+  *gb = -((act1 ? g_im : 0.f) + (act2 ? g_im2 : 0.f));
 }
 
 // the block's threads (NWV waves, one codeword) at a layer boundary
