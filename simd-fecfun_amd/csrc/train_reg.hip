@@ -257,9 +257,9 @@ __device__ __forceinline__ void load_state(uint32_t (&w)[kStateWords<QT>], const
 // layer 0: layer mrow - 1 of iteration t - 1 at Snext, if not null), which it leaves in
 // cur.  Writes the codeword's parameter gradients: per (iteration, layer) the sum over
 // lanes l of (row l's sum [+ row l + 64's sum]), each row's terms summed from 0 in edge
-// order -- train.hip's order.  With two waves, the second hands its rows' sums to the
-// first through xch: two slots, alternating over the layers in the order they are swept,
-// so the second wave's next write never meets the first wave's read of the last.
+// order -- train.hip's order.  With two waves each sums one of the two, the other handing
+// it its rows' terms through xch: two slots, alternating over the layers in the order
+// they are swept, so a wave's next write never meets the other wave's read of the last.
 template <class SH, int L, int NWV, bool QT>
 __device__ __forceinline__ void rev_layers(const TrainArgs& a, const uint32_t* tab, float (&g)[SH::ne],
                                            const float* St, const float* Snext, uint32_t (&cur)[kStateWords<QT>],
@@ -279,21 +279,23 @@ __device__ __forceinline__ void rev_layers(const TrainArgs& a, const uint32_t* t
     asm volatile("" : "+v"(r), "+v"(r4), "+v"(rz4));
     if (r < a.z) rev_row_reg<W, QT>(tw, &g[E0], cur, r4, rz4, al, cap, &ga, &gb);
     if constexpr (NWV == 2) {
+      // the first wave sums the scale's terms, the second the offset's: each hands the
+      // other its rows' terms of the other sum
       float* x = xch + ((t * SH::mrow + L) & 1) * 128;
-      if (row >= 64) {
-        x[row - 64] = ga;
-        x[row] = gb;
+      if (row < 64) x[64 + row] = gb;
+      else x[row - 64] = ga;
+      block_sync<NWV>();
+      if (row < 64) {
+        if (row + 64 < a.z) ga = ga + x[row];
+        ga = wave_sum(ga);
+        if (row == 0) part[(t * SH::mrow + L) * 2 + 0] = ga;
+      } else {
+        gb = row < a.z ? x[row] + gb : x[row];
+        gb = wave_sum(gb);
+        if (row == 64) part[(t * SH::mrow + L) * 2 + 1] = gb;
       }
-    }
-    block_sync<NWV>();
-    if (row < 64) {   // the first wave, all its lanes
-      if constexpr (NWV == 2) {
-        const float* x = xch + ((t * SH::mrow + L) & 1) * 128;
-        if (row + 64 < a.z) {
-          ga = ga + x[row];
-          gb = gb + x[row + 64];
-        }
-      }
+    } else {
+      block_sync<NWV>();
       ga = wave_sum(ga);
       gb = wave_sum(gb);
       if (row == 0) {
