@@ -422,8 +422,9 @@ void fecsim_msim_free(fecsim_msim *s);
  * 802.16e), each row's two smallest magnitudes and a word of indices and per-edge flags
  * (S = 3 mrow z, 4 mrow z with FECSIM_I8).  K is the segment length in 1..T of least
  * memory ceil(T/K) (N + E) + K S (the larger on a tie), of order sqrt(T (N + E) / S).
- * Device memory per codeword: ceil(T/K) (N + E) + K S + 2 E values, plus 2 N when the
- * posteriors sit in global memory (N = mcol z variables, E = total_hc_ones z edges). */
+ * Device memory per codeword: ceil(T/K) (N + E) + K S + 2 E values (E instead of 2 E with the
+ * register kernels), plus 2 N when the posteriors sit in global memory (N = mcol z variables, E =
+ * total_hc_ones z edges). */
 typedef struct fecsim_trainer fecsim_trainer;
 
 typedef struct {
