@@ -12,6 +12,8 @@
 //                    one code) from an atomic counter until the queue is drained
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "decode_core.h"
 #include "internal.h"
 
@@ -72,8 +74,17 @@ __global__ void __launch_bounds__(256) k_multi_scatter(MultiArgs a) {
   }
 }
 
+// int8 rows: at most 168 VGPRs, 3 waves per SIMD (the weight-24 instance takes 197 and
+// 2 waves unbounded; bounded, its heaviest rows spill a few registers and the mixed
+// queue of config 3 decodes 10% faster)
+template <class P>
+constexpr int multi_min_waves() {
+  return std::is_same<P, IntPolicy<8>>::value ? 3 : 1;
+}
+
 template <class P, int MAXW>
-__global__ void __launch_bounds__(256) k_multi_decode(const MultiArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(multi_min_waves<P>())))
+k_multi_decode(const MultiArgs a) {
   using S = typename P::S;
   HIP_DYNAMIC_SHARED(char, smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
