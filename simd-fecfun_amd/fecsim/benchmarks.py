@@ -7,7 +7,8 @@
      11n 3/4 z54, 11n 5/6 z81, 11ad 13/16, 16e 2/3A Z=96, Eb/N0 uniform in [1, 4] dB,
      int8 offset min-sum, per-code max iterations 8/12/16/20/10, autotuned tile shapes,
      one HIP graph per loop (encode + WGN/LLR + decode + count)
-  4  trainable decoder: 11n 1/2 z81, T = 10, batch 4096, forward + adjoint per step
+  4  trainable decoder: 11n 1/2 z81, T = 10, batch 4096, forward + adjoint per step; then 400
+     training steps and the learned tables against the plain float and int8 decoders
   5  5G NR base graph 1, Z = 384, rate 1/3 (2Z punctured), int8 offset min-sum, max 20
      iterations, 16384-codeword batches, Eb/N0 0..1.5 dB; throughput at CWER 1e-3
 
@@ -236,13 +237,51 @@ def config4(args):
         kf += ev[0].elapsed_time(ev[1])
         kb += ev[1].elapsed_time(ev[2])
     E = cs.dec.total_hc_ones * cs.z
-    return {"config": 4, "workload": "11n 1/2 z81 trainable offset/normalized min-sum, T=10, batch 4096, "
+    procedure = config4_procedure(cs, henc) if getattr(args, "procedure", True) else None
+    return {"config": 4, "procedure": procedure,
+            "workload": "11n 1/2 z81 trainable offset/normalized min-sum, T=10, batch 4096, "
                                      "forward + adjoint (checkpoints at segment starts, segment length of least memory, "
                                      "segments recomputed)",
             "steps": steps, "step_ms": dt / steps * 1e3, "forward_ms": fwd_ms / steps, "backward_ms": bwd_ms / steps,
             "decoder_forward_ms": kf / steps, "decoder_adjoint_ms": kb / steps, "adjoint_over_forward": kb / kf,
             "cw_per_s_train": B * steps / dt, "edge_updates_per_s_forward": B * T * E / (kf / steps * 1e-3),
             "dtype": "f32", "data": "synthetic"}
+
+
+def config4_procedure(cs, henc, steps=400, ncw=1 << 18):
+    """The training procedure of the config: Adam (lr 1e-3) on the BCE of the posteriors
+    after T = 10 iterations, scales and offsets per (iteration, layer) from 0.8 and 0.5,
+    batches of 4096 codewords at Eb/N0 U[1, 3] dB; then the learned float decoder against
+    the plain one (offset 0.5, scale 1), and the learned offsets quantised into the int8
+    decoder (LLR scale 2) against its plain offset 1, both at 10 iterations, CWER 1e-3
+    crossings on the same codewords and noise."""
+    from .decoder import Decoder, Hdec
+    from .train import cwer_curve, eval_float, quantize_offsets, snr_at, train
+    T = 10
+    t0 = time.perf_counter()
+    alpha, beta, losses = train(cs, henc, T=T, batch=4096, steps=steps, ebn0=(1.0, 3.0), lr=1e-3)
+    t_train = time.perf_counter() - t0
+    snrs = [-1.5 + 0.125 * i for i in range(17)]
+    ones = np.ones_like(alpha)
+    fp = eval_float(cs, henc, ones, 0.5 * ones, snrs, ncw, T)
+    fl = eval_float(cs, henc, alpha, beta, snrs, ncw, T)
+    q = quantize_offsets(beta, 2.0, max_iter=T)
+    hdec = Hdec.from_tables(cs.dec)
+    plain = Decoder(hdec, "Vec64c_min", T, 31.0, 1.0)
+    learned = Decoder(hdec, "Vec64c_min", T, 31.0, 1.0)
+    learned.set_offsets(q)
+    ip = cwer_curve(cs, henc, plain, snrs, ncw)
+    il = cwer_curve(cs, henc, learned, snrs, ncw)
+
+    def gain(a, b):
+        sa, sb = snr_at(a), snr_at(b)
+        return (sa - sb) if sa is not None and sb is not None else None
+
+    return {"steps": steps, "train_seconds": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+            "float_gain_db": gain(fp, fl), "int8_gain_db": gain(ip, il),
+            "int8_table_entries": sorted({int(x) for x in q.ravel()}), "cw_per_point": ncw, "es_n0_db": snrs,
+            "cwer_float_plain": [c for _, c, *_ in fp], "cwer_float_learned": [c for _, c, *_ in fl],
+            "cwer_int8_plain": [c for _, c, *_ in ip], "cwer_int8_learned": [c for _, c, *_ in il]}
 
 
 def bg1_learned_gain(cs=None, henc=None, plain=None):
