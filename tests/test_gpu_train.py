@@ -340,12 +340,15 @@ def test_gpu_train_backward_keeps_its_parameters(cuda):
 @pytest.mark.parametrize("name,T,cap,int8", [("11n 1/2 z81", 10, 6.0, False), ("11n 1/2 z54", 7, 1e30, False),
                                             ("11n 1/2 z27", 4, 1e30, False), ("11ad 1/2", 9, 1e30, False),
                                             ("16e 1/2", 5, 8.0, False), ("11ad 1/2", 6, 31.0, True),
-                                            ("11n 1/2 z81", 5, 31.0, True)])
+                                            ("11n 1/2 z81", 5, 31.0, True), ("11ad 1/2", 1, 1e30, False),
+                                            ("11ad 1/2", 2, 4.0, False), ("11n 1/2 z81", 1, 1e30, False),
+                                            ("11n 1/2 z81", 20, 6.0, False), ("11ad 1/2", 13, 31.0, True)])
 def test_gpu_train_register_kernels_match_global_c2v(name, T, cap, int8, cuda):
-    """The register-C2V kernels (compiled shapes, one wave per codeword) against the
-    global-C2V kernels forced by FECSIM_TRAIN_GLOBAL_C2V: posteriors and all three
-    gradients equal bit for bit, over several checkpoint segments (T >= 5 recomputes)
-    and a batch that is not a multiple of the global kernels' waves per block."""
+    """The register-C2V kernels (compiled shapes, one or two waves per codeword) against
+    the global-C2V kernels forced by FECSIM_TRAIN_GLOBAL_C2V: posteriors and all three
+    gradients equal bit for bit, from one segment (T = 1, 2: nothing recomputed) to several
+    (T = 20: segments of 10), and a batch that is not a multiple of the global kernels'
+    waves per block."""
     import torch
     from fecsim.train import Trainer
     cs = _cs(name)
