@@ -152,9 +152,17 @@ __device__ __forceinline__ bool parity(const typename P::S* post, const uint32_t
   }
 }
 
+// shapes of up to 76 edges (802.16e 1/2) bounded to 128 VGPRs, 4 waves per SIMD: unbounded
+// they take 133-136 and run 3 (a few registers spill; the 16e 1/2 sweep decodes 7% faster)
+template <class SH>
+constexpr int float_reg_min_waves() {
+  return SH::ne <= 76 ? 4 : 1;
+}
+
 // core.cw_per_wave carries CPB (codewords per block) and core.post_stride the padded N
 template <class P, class SH>
-__global__ void __launch_bounds__(256) k_float_reg(const KernelArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(float_reg_min_waves<SH>())))
+k_float_reg(const KernelArgs a) {
   using F = typename P::S;
   HIP_DYNAMIC_SHARED(char, smem);
   const DecCore& d = a.core;
