@@ -436,8 +436,12 @@ hipError_t launch_llr(int elem, const LlrArgs& a0, hipStream_t st) {
     case FECSIM_F32: fn = (const void*)k_llr<float>; break;
     default: fn = (const void*)k_llr<double>; break;
     }
-    const uint64_t cap = (uint64_t)resident_blocks(fn, 256, 0) * 8;
-    if (grid > cap) grid = cap;
+    // at most 8 rounds of resident blocks; but at least 16 pairs a thread beyond one round:
+    // every block adds its uncoded-error count to one counter as it ends, and in a short
+    // launch the blocks end together, so their atomics queue (~4 ns each)
+    const uint64_t res = (uint64_t)resident_blocks(fn, 256, 0);
+    const uint64_t lean = (threads + 256 * 16 - 1) / (256 * 16);
+    grid = std::min(grid, std::min(res * 8, std::max(res, lean)));
     switch (elem) {
     case FECSIM_I8: hipLaunchKernelGGL(k_llr<int8_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
     case FECSIM_I16: hipLaunchKernelGGL(k_llr<int16_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
@@ -461,9 +465,11 @@ hipError_t launch_count(int elem, const CountArgs& a0, hipStream_t st) {
   a.div_dz = make_fastdiv(dz);
   a.div_z = make_fastdiv((uint32_t)a.z);
   const uint64_t tiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
-  uint64_t grid = tiles;
-  const uint64_t cap = (uint64_t)resident_blocks((const void*)k_count<int8_t>, 256, 0) * 4;
-  if (grid > cap) grid = cap;
+  // at most 4 rounds of resident blocks; but at least 8 tiles a block beyond one round (each
+  // block's two counter atomics at its end queue when the blocks of a short launch end
+  // together)
+  const uint64_t res = (uint64_t)resident_blocks((const void*)k_count<int8_t>, 256, 0);
+  const uint64_t grid = std::min(tiles, std::min(res * 4, std::max(res, (tiles + 7) / 8)));
   switch (elem) {
   case FECSIM_I8: hipLaunchKernelGGL(k_count<int8_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
   case FECSIM_I16: hipLaunchKernelGGL(k_count<int16_t>, dim3((uint32_t)grid), dim3(256), 0, st, a); break;
