@@ -174,7 +174,12 @@ int fecsim_sp_math(int fn, const float *x, float *y, uint32_t n, void *stream);
 /* One persistent launch decodes a queue whose codewords belong to different codes:
  * decoder c (its tables, max_iter, beta, cap) decodes every queue entry with code id
  * c, with results identical to fecsim_decode on that decoder.  All decoders need the
- * same element kind and mode, and their messages must fit one wave's LDS slice. */
+ * same element kind and mode, and their messages must fit one wave's LDS slice.
+ * Two kernels serve a queue: the generic one (C2V messages in LDS, any codes) and,
+ * when every decoder's code is compiled into it (802.11n 1/2 z27, 3/4 z54, 5/6 z81,
+ * 802.11ad 13/16, 802.16e 2/3A at Z = 96; int8 min-sum, 1 <= beta, 0 <= cap, offset
+ * tables with entries 1..127), one with the codes compiled in (C2V messages in
+ * registers); the latter is the default then, and both give identical results. */
 typedef struct fecsim_multi fecsim_multi;
 
 int fecsim_multi_create(fecsim_decoder *const *decs, int ncode, uint32_t max_cw, fecsim_multi **out);
@@ -186,12 +191,21 @@ int fecsim_multi_decode(fecsim_multi *m, uint32_t num_cw, const int32_t *code, c
                         const void *llr_in, void *post_out, uint32_t *iters, unsigned long long *tot_iter,
                         void *stream);
 /* Autotuner: times the queue (decoded into post_out, best of `rounds`) for candidate
- * tile shapes -- rows per lane, hence codewords per wave, for each code in turn, then
- * waves per block -- and keeps the fastest.  Synchronises the stream. */
+ * tile shapes of the generic kernel -- rows per lane, hence codewords per wave, for each
+ * code in turn, then waves per block -- and keeps the fastest; then, when the codes are
+ * compiled in and the kernel choice is FECSIM_KERNEL_AUTO, times that kernel and runs
+ * the queue on whichever of the two is faster.  With FECSIM_KERNEL_CODE selected it
+ * does nothing.  Synchronises the stream. */
 int fecsim_multi_autotune(fecsim_multi *m, uint32_t num_cw, const int32_t *code, const unsigned long long *off,
                           const void *llr_in, void *post_out, int rounds, void *stream);
 int fecsim_multi_shape(const fecsim_multi *m, int code, int *rows_per_lane, int *cw_per_wave,
                        int *waves_per_block);
+/* kernel: FECSIM_KERNEL_AUTO (the compiled-code kernel when every code is compiled in),
+ * FECSIM_KERNEL_GENERIC, or FECSIM_KERNEL_CODE (FECSIM_E_UNSUPPORTED when a code is not
+ * compiled in).  fecsim_multi_kernel: FECSIM_KERNEL_GENERIC or FECSIM_KERNEL_CODE, the
+ * kernel fecsim_multi_decode runs now. */
+int fecsim_multi_select_kernel(fecsim_multi *m, int kernel);
+int fecsim_multi_kernel(const fecsim_multi *m);
 /* fecsim_multi_decode without any allocation, for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops that forbid hipMalloc): the queue
  * workspace (grouping permutation, per-code counters, work counter) is a static device

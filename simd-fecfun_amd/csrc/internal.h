@@ -191,9 +191,11 @@ struct MultiArgs {
   uint32_t* code_start;      // [ncode + 1]
   uint32_t* item_start;      // [ncode + 1]
   uint32_t* counter;         // next work item
+  uint32_t* take;            // [ncode] codewords of each code's segment taken (k_multi_code)
   int wave_bytes, waves;     // LDS per wave (max over codes), waves per block
   const uint32_t* tabs;      // every code's [edges | layer_w], copied to the front of LDS
   int tab_words, tab_bytes;
+  const int32_t* ctab;       // [ncode] compiled code of each decoder (k_multi_code)
 };
 
 // k_pair_i8c's counter sets per decoder: that many of its decodes may overlap on the device
@@ -262,7 +264,14 @@ hipError_t prepare_generic(const DecoderDev& d);
 void offsets_table(const DecoderDev& d, const float* beta, size_t n, std::vector<long long>* out);
 // mixed-code decoder: blocks resident for its kernel, and the launch (sort + decode)
 int multi_resident_blocks(int elem, int mode, int maxw, int waves, size_t lds);
-hipError_t launch_multi(int elem, int mode, int maxw, const MultiArgs& a, uint32_t grid, hipStream_t stream);
+// code_kernel: k_multi_code (every decoder's multi_code_index >= 0, a.ctab holds them)
+hipError_t launch_multi(int elem, int mode, int maxw, const MultiArgs& a, uint32_t grid, hipStream_t stream,
+                        bool code_kernel);
+// the queue kernel with the codes compiled in: decoder d's compiled code or -1, its LDS
+// per workgroup and resident workgroups on the current device
+int multi_code_index(const DecoderDev& d);
+size_t multi_code_lds();
+int multi_code_resident_blocks();
 // a DecShape with `rows_per_lane` rows per lane (0: the best lane utilisation), C2V in LDS;
 // returns non-zero when it does not fit
 int shape_with_rows(const HdecHost& h, int elem_bytes, int rows_per_lane, DecShape* s);
@@ -342,10 +351,15 @@ struct fecsim_multi {
   int waves = 4, wave_bytes = 0;          // in use
   uint32_t grid = 0, max_cw = 0;
   fecsim::DecCore* d_cores = nullptr;
-  uint32_t* d_ws = nullptr;               // perm | cnt | cursor | code_start | item_start | counter
+  uint32_t* d_ws = nullptr;               // perm | cnt | cursor | code_start | item_start | counter | take
   uint32_t* d_tabs = nullptr;             // all codes' [edges | layer_w]
   std::vector<int> tab_off;               // word offset of each code's tables
   int tab_words = 0, tab_bytes = 0;
+  int32_t* d_ctab = nullptr;              // [ncode] compiled code per decoder (k_multi_code)
+  bool code_ok = false;                   // every decoder's code is compiled into k_multi_code
+  int kernel_pref = FECSIM_KERNEL_AUTO;   // FECSIM_KERNEL_AUTO, _GENERIC or _CODE
+  bool use_code = false;                  // k_multi_code runs the queue
+  uint32_t code_grid = 0;                 // its persistent grid
 };
 
 namespace fecsim {

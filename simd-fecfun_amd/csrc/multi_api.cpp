@@ -17,11 +17,11 @@ namespace {
 
 // Workspace of fecsim_multi_decode_static: the queue permutation (FECSIM_MULTI_STATIC_MAX_CW
 // entries) and the per-code counters (histogram, scatter cursors, codeword and work-item
-// starts for up to 64 codes, the work counter), 4 MiB + 1036 bytes.  Every GPU the
-// library runs on holds its own copy until the process ends; all static calls on one GPU
-// share it.
+// starts for up to 64 codes, the work counter, codewords taken per code), 4 MiB + 1292
+// bytes.  Every GPU the library runs on holds its own copy until the process ends; all
+// static calls on one GPU share it.
 constexpr uint32_t kStaticMaxCw = FECSIM_MULTI_STATIC_MAX_CW;
-constexpr size_t kStaticWords = (size_t)kStaticMaxCw + 64 + 64 + 65 + 65 + 1;
+constexpr size_t kStaticWords = (size_t)kStaticMaxCw + 64 + 64 + 65 + 65 + 1 + 64;
 static __device__ __align__(256) uint32_t scratch[kStaticWords];
 
 // Device address of `scratch` per device, looked up once with that device current (in host
@@ -50,6 +50,7 @@ void free_multi(fecsim_multi* m) {
   (void)hipFree(m->d_cores);
   (void)hipFree(m->d_ws);
   (void)hipFree(m->d_tabs);
+  (void)hipFree(m->d_ctab);
 }
 
 // recompute the per-code cores, the LDS slice and the persistent grid for m->shapes
@@ -97,6 +98,8 @@ MultiArgs args(const fecsim_multi* m, uint32_t* ws, uint32_t ws_cw, uint32_t num
   a.code_start = a.cursor + n;
   a.item_start = a.code_start + n + 1;
   a.counter = a.item_start + n + 1;
+  a.take = a.counter + 1;
+  a.ctab = m->d_ctab;
   a.wave_bytes = m->wave_bytes;
   a.waves = m->waves;
   a.tabs = m->d_tabs;
@@ -109,12 +112,17 @@ int run(fecsim_multi* m, uint32_t* ws, uint32_t ws_cw, uint32_t num_cw, const in
         const unsigned long long* off, const void* in, void* out, uint32_t* iters, unsigned long long* tot,
         hipStream_t st) {
   const MultiArgs a = args(m, ws, ws_cw, num_cw, code, off, in, out, iters, tot);
+  if (m->use_code) {   // persistent grid; a workgroup without codewords leaves at once
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(m->code_grid, num_cw));
+    const hipError_t e = launch_multi(m->elem, m->mode, m->maxw, a, grid, st, true);
+    return e == hipSuccess ? FECSIM_OK : hip_fail(e, "multi-code decode launch");
+  }
   // persistent grid, no larger than the work could use
   uint32_t min_cpw = 64;
   for (const DecShape& s : m->shapes) min_cpw = std::min<uint32_t>(min_cpw, (uint32_t)s.cw_per_wave);
   const uint64_t need = ((uint64_t)num_cw + (uint64_t)min_cpw * m->waves - 1) / ((uint64_t)min_cpw * m->waves);
   const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(m->grid, need));
-  const hipError_t e = launch_multi(m->elem, m->mode, m->maxw, a, grid, st);
+  const hipError_t e = launch_multi(m->elem, m->mode, m->maxw, a, grid, st, false);
   return e == hipSuccess ? FECSIM_OK : hip_fail(e, "multi-code decode launch");
 }
 
@@ -194,18 +202,34 @@ int fecsim_multi_create(fecsim_decoder* const* decs, int ncode, uint32_t max_cw,
     delete m;
     return fail(FECSIM_E_UNSUPPORTED, "the codes' edge tables exceed 32 KB of LDS");
   }
+  // the queue kernel with the codes compiled in, when every decoder's code is
+  std::vector<int32_t> ctab(ncode);
+  m->code_ok = true;
+  for (int c = 0; c < ncode; ++c) {
+    ctab[c] = multi_code_index(m->decs[c]->d);
+    m->code_ok = m->code_ok && ctab[c] >= 0;
+  }
+  const size_t ws_words = (size_t)max_cw + 5 * (size_t)ncode + 3;
   hipError_t e = hipMalloc((void**)&m->d_cores, sizeof(DecCore) * ncode);
-  if (e == hipSuccess) e = hipMalloc((void**)&m->d_ws, sizeof(uint32_t) * ((size_t)max_cw + 4 * (size_t)ncode + 3));
-  if (e == hipSuccess) e = hipMemset(m->d_ws, 0, sizeof(uint32_t) * ((size_t)max_cw + 4 * (size_t)ncode + 3));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_ws, sizeof(uint32_t) * ws_words);
+  if (e == hipSuccess) e = hipMemset(m->d_ws, 0, sizeof(uint32_t) * ws_words);
   if (e == hipSuccess) e = hipMalloc((void**)&m->d_tabs, sizeof(uint32_t) * tabs.size());
   if (e == hipSuccess)
     e = hipMemcpy(m->d_tabs, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_ctab, sizeof(int32_t) * ncode);
+  if (e == hipSuccess) e = hipMemcpy(m->d_ctab, ctab.data(), sizeof(int32_t) * ncode, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_multi(m);
     delete m;
     return hip_fail(e, "multi-code buffers");
   }
-  const int rc = update(m);
+  int rc = update(m);
+  if (!rc && m->code_ok) {
+    const int blocks = multi_code_resident_blocks();
+    if (blocks < 1) rc = fail(FECSIM_E_HIP, "occupancy query failed");
+    m->code_grid = (uint32_t)blocks;
+    m->use_code = true;
+  }
   if (rc) {
     free_multi(m);
     delete m;
@@ -249,8 +273,10 @@ int fecsim_multi_autotune(fecsim_multi* m, uint32_t num_cw, const int32_t* code,
   if (num_cw > m->max_cw) return fail(FECSIM_E_INVAL, "queue longer than max_cw");
   hipStream_t st = (hipStream_t)stream;
   if (rounds < 1) rounds = 1;
+  if (m->kernel_pref == FECSIM_KERNEL_CODE) return FECSIM_OK;   // nothing to tune
   int rc = update(m);
   if (rc) return rc;
+  m->use_code = false;   // the generic kernel's shapes first
   double best = time_queue(m, num_cw, code, off, llr_in, post_out, rounds, st);
   if (best < 0) return fail(FECSIM_E_HIP, "autotune timing failed");
   // coordinate descent: rows per lane (hence codewords per wave) of every code, then waves per block
@@ -287,7 +313,31 @@ int fecsim_multi_autotune(fecsim_multi* m, uint32_t num_cw, const int32_t* code,
     }
   }
   m->waves_pref = keep_w;
-  return update(m);
+  if ((rc = update(m))) return rc;
+  // then the kernel: the compiled-code one when it is faster than the tuned generic one
+  if (m->code_ok && m->kernel_pref == FECSIM_KERNEL_AUTO) {
+    m->use_code = true;
+    const double t = time_queue(m, num_cw, code, off, llr_in, post_out, rounds, st);
+    if (t < 0) return fail(FECSIM_E_HIP, "autotune timing failed");
+    m->use_code = t < best;
+  }
+  return FECSIM_OK;
+}
+
+int fecsim_multi_select_kernel(fecsim_multi* m, int kernel) {
+  if (!m) return fail(FECSIM_E_INVAL, "null argument");
+  if (kernel != FECSIM_KERNEL_AUTO && kernel != FECSIM_KERNEL_GENERIC && kernel != FECSIM_KERNEL_CODE)
+    return fail(FECSIM_E_INVAL, "kernel must be FECSIM_KERNEL_AUTO, _GENERIC or _CODE");
+  if (kernel == FECSIM_KERNEL_CODE && !m->code_ok)
+    return fail(FECSIM_E_UNSUPPORTED, "a code of the queue is not compiled into the queue kernel");
+  m->kernel_pref = kernel;
+  m->use_code = m->code_ok && kernel != FECSIM_KERNEL_GENERIC;
+  return FECSIM_OK;
+}
+
+int fecsim_multi_kernel(const fecsim_multi* m) {
+  if (!m) return fail(FECSIM_E_INVAL, "null argument");
+  return m->use_code ? FECSIM_KERNEL_CODE : FECSIM_KERNEL_GENERIC;
 }
 
 int fecsim_multi_shape(const fecsim_multi* m, int c, int* rows_per_lane, int* cw_per_wave, int* waves_per_block) {

@@ -150,7 +150,8 @@ def config3(args):
     _L.check(_L.lib().fecsim_multi_autotune(m._m, n, code, off, llr, C.c_void_p(scratch.data_ptr()), 3,
                                             C.c_void_p(torch.cuda.current_stream().cuda_stream)), "autotune")
     shapes1 = [m.shape(c) for c in range(len(CODES3))]
-    # decode-only launch of the tuned queue for the roofline (k_multi_decode, sort kernels included)
+    kernel = m.kernel   # the autotuner's choice: the compiled-code kernel or the tuned generic one
+    # decode-only launches of the tuned queue for the roofline (grouping sort included)
     qcode, qoff, _, _ = tune.queue()
     it = torch.zeros(n, dtype=torch.int32, device="cuda")
     st_ = torch.cuda.current_stream()
@@ -163,8 +164,17 @@ def config3(args):
     ms = _timed(_dec, st_, reps=3)
     e_code = torch.tensor([cs.dec.total_hc_ones * cs.z for cs in css], dtype=torch.float64, device="cuda")
     eu = float((it.double() * e_code[qcode.long()]).sum().item())
-    roof = _roofline("k_multi_decode<IntPolicy<8>,24>", eu, ms, OPS_OMS,
-                     "decode-only launches of the autotuned 65536-entry queue (grouping sort included)")
+    code_kernel = kernel == _L.KERNEL_CODE
+    roof = _roofline("k_multi_code (codes compiled in)" if code_kernel else "k_multi_decode<IntPolicy<8>,24>", eu, ms,
+                     OPS_OMS, "decode-only launches of the autotuned 65536-entry queue (grouping sort included)")
+    # the other kernel on the same queue, for comparison; then the tuned choice again
+    try:
+        m.select_kernel(_L.KERNEL_GENERIC if code_kernel else _L.KERNEL_CODE)
+        _dec()
+        other_ms = _timed(_dec, st_, reps=3)
+    except _L.FecsimError:   # a code not compiled in: the generic kernel only
+        other_ms = None
+    m.select_kernel(_L.KERNEL_CODE if code_kernel else _L.KERNEL_GENERIC)
     del tune
     sim = MixedSim(hencs, m, n, ebn0=(1.0, 4.0), seed=100, use_graph=True)
     sim.run(1)
@@ -176,7 +186,9 @@ def config3(args):
     st, iters = sim.stats()
     dt = time.perf_counter() - t0
     info_bits = sum(s["cw"] * (cs.enc.mcol - cs.enc.mrow) * cs.z for s, cs in zip(st, css))
-    return {"config": 3, "roofline": roof, "workload": "mixed queue of 65536 cw per loop over 5 codes (11n 1/2 z27, 11n 3/4 z54, "
+    return {"config": 3, "roofline": roof, "kernel": "k_multi_code" if code_kernel else "k_multi_decode",
+            "decode_only_ms": ms, ("decode_only_ms_generic" if code_kernel else "decode_only_ms_code"): other_ms,
+            "workload": "mixed queue of 65536 cw per loop over 5 codes (11n 1/2 z27, 11n 3/4 z54, "
                                      "11n 5/6 z81, 11ad 13/16, 16e 2/3A Z=96), Eb/N0 U[1,4] dB, int8 min-sum, max "
                                      "iterations 8/12/16/20/10, HIP graph per loop (encode+WGN+decode+count)",
             "loops": loops, "seconds": dt, "cw_per_s": n * loops / dt, "info_gbit_s": info_bits / dt / 1e9,

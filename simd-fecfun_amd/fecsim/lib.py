@@ -63,6 +63,8 @@ SIGNATURES = [
     ("fecsim_sp_math", _int, [_int, _vp, _vp, _u32, _vp]),
     ("fecsim_multi_autotune", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _int, _vp]),
     ("fecsim_multi_shape", _int, [_vp, _int, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
+    ("fecsim_multi_select_kernel", _int, [_vp, _int]),
+    ("fecsim_multi_kernel", _int, [_vp]),
     ("fecsim_multi_free", None, [_vp]),
     ("fecsim_msim_create", _int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     ("fecsim_msim_run", _int, [_vp, _u32, _vp]),

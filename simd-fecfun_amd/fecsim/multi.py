@@ -47,6 +47,18 @@ class MultiDecoder:
         _L.check(_L.lib().fecsim_multi_shape(self._m, c, C.byref(k), C.byref(cpw), C.byref(w)), "fecsim_multi_shape")
         return k.value, cpw.value, w.value
 
+    def select_kernel(self, kernel: int) -> None:
+        """lib.KERNEL_AUTO, KERNEL_GENERIC or KERNEL_CODE (the codes compiled in)."""
+        _L.check(_L.lib().fecsim_multi_select_kernel(self._m, kernel), "fecsim_multi_select_kernel")
+
+    @property
+    def kernel(self) -> int:
+        """lib.KERNEL_GENERIC or KERNEL_CODE: the kernel decode() runs now."""
+        k = _L.lib().fecsim_multi_kernel(self._m)
+        if k < 0:
+            _L.check(k, "fecsim_multi_kernel")
+        return k
+
     def __del__(self):
         if getattr(self, "_m", None) and _L._lib is not None:
             _L.lib().fecsim_multi_free(self._m)
