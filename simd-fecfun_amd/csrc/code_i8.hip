@@ -11,7 +11,7 @@
 // * A workgroup barrier separates two layers that share a column (a compile-time plan;
 //   base graph 1 needs 32 per iteration for its 46 layers), and reads of columns not
 //   written since the last barrier are issued before it.  The syndrome after each
-//   iteration is the XOR of each row's biased bytes, and a workgroup OR ends the loop
+//   iteration is the XOR of each row's posterior bytes, and a workgroup OR ends the loop
 //   (src/LDPCDec/LDPCdecode.cpp:172-197).
 #include <mutex>
 
@@ -34,8 +34,19 @@ struct CodeArgs {
 
 namespace {
 
+// the row arithmetic's form per code (code_core.h code_layer): two edges per operation for
+// codes whose layers are heavy on average (802.3ca, 22.9 edges: 7.70 -> 7.02 ms per 32768
+// codewords), one for the others (base graph 1, 6.9 edges over 46 mostly short layers:
+// 17.8 against 18.9 ms per 16384)
+template <class CT>
+constexpr bool code_i8_packed() {
+  return CT::ne >= 12 * CT::mrow;
+}
+
 template <class CT>
 __global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(4))) k_code_i8(const CodeArgs a) {
+  constexpr bool PK = code_i8_packed<CT>();
+  constexpr uint32_t BIAS = PK ? 0u : 0x80808080u;   // posteriors in LDS: int8 or biased by 128
   constexpr int Z = CT::z, Q = Z / 4, NE = CT::ne, NW = (NE + 3) / 4, MC = CT::mcol;
   constexpr int LDS = (3 * MC + 1) * Z;
   constexpr int NB = (LDS + 32767) / 32768;   // base registers, one per 32 KiB: offsets stay < 32768
@@ -54,14 +65,14 @@ __global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(4)))
   unsigned long long tot = 0;
 
   for (long long cw = blockIdx.x; cw < (long long)a.num_cw; cw += gridDim.x) {   // uniform per block
-    // ---- codeword in: both copies of each column, biased ----
+    // ---- codeword in: both copies of each column ----
     const uint4* gi = reinterpret_cast<const uint4*>(a.llr_in + cw * (long long)(MC * Z));
     for (int k = i; k < CH; k += Z) {
       uint4 v = gi[k];
-      v.x ^= 0x80808080u;
-      v.y ^= 0x80808080u;
-      v.z ^= 0x80808080u;
-      v.w ^= 0x80808080u;
+      v.x ^= BIAS;
+      v.y ^= BIAS;
+      v.z ^= BIAS;
+      v.w ^= BIAS;
       const int c = k / CPC, o = (k - c * CPC) * 16;
       *reinterpret_cast<uint4*>(smem + Z + 3 * Z * c + o) = v;
       *reinterpret_cast<uint4*>(smem + 2 * Z + 3 * Z * c + o) = v;
@@ -75,8 +86,8 @@ __global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(4)))
     for (;;) {
       const long long* bt = a.beta_tab ? a.beta_tab + (size_t)iter * CT::mrow : nullptr;
       int pin0[CT::w[0]] = {};   // layer 0 reads nothing ahead
-      code_layers<CT, 0, 0>(vb, c2v, bt, (uint32_t)a.beta, a.cap, pin0, true);
-      const uint32_t F = code_syndrome<CT, 0, 0>(vb);   // syndrome after the full iteration
+      code_layers<CT, 0, 0, PK>(vb, c2v, bt, (uint32_t)a.beta, a.cap, pin0, true);
+      const uint32_t F = code_syndrome<CT, 0, 0, PK>(vb);   // syndrome after the full iteration
       ++iter;
       const int any = __syncthreads_or((int)(F & 0x80u));
       if (!any || iter == a.max_iter) break;
@@ -87,10 +98,10 @@ __global__ void __launch_bounds__(CT::z) __attribute__((amdgpu_waves_per_eu(4)))
     for (int k = i; k < CH; k += Z) {
       const int c = k / CPC, o = (k - c * CPC) * 16;
       uint4 v = *reinterpret_cast<const uint4*>(smem + Z + 3 * Z * c + o);
-      v.x ^= 0x80808080u;
-      v.y ^= 0x80808080u;
-      v.z ^= 0x80808080u;
-      v.w ^= 0x80808080u;
+      v.x ^= BIAS;
+      v.y ^= BIAS;
+      v.z ^= BIAS;
+      v.w ^= BIAS;
       go[k] = v;
     }
     if (a.c2v) {

@@ -180,7 +180,8 @@ constexpr int mc_lds_max() {
   }
 }
 
-// row r of every column of a codeword into both LDS copies (biased by 128), and back
+// row r of every column of a codeword into both LDS copies (int8: the two-edge form of the
+// row arithmetic), and back
 template <class CT>
 __device__ __forceinline__ void mc_load(uint32_t vb, const int8_t* __restrict__ g) {
   constexpr int Z = CT::z, MC = CT::mcol;
@@ -189,8 +190,8 @@ __device__ __forceinline__ void mc_load(uint32_t vb, const int8_t* __restrict__ 
   for (int c = 0; c < MC; ++c) v[c] = (uint8_t)g[c * Z];
 #pragma unroll
   for (int c = 0; c < MC; ++c) {
-    *lds8(vb + (uint32_t)(Z + 3 * Z * c)) = (uint8_t)(v[c] ^ 0x80u);
-    *lds8(vb + (uint32_t)(2 * Z + 3 * Z * c)) = (uint8_t)(v[c] ^ 0x80u);
+    *lds8(vb + (uint32_t)(Z + 3 * Z * c)) = (uint8_t)v[c];
+    *lds8(vb + (uint32_t)(2 * Z + 3 * Z * c)) = (uint8_t)v[c];
   }
 }
 
@@ -201,7 +202,7 @@ __device__ __forceinline__ void mc_store(int8_t* __restrict__ g, uint32_t vb) {
 #pragma unroll
   for (int c = 0; c < MC; ++c) v[c] = *lds8(vb + (uint32_t)(Z + 3 * Z * c));
 #pragma unroll
-  for (int c = 0; c < MC; ++c) g[c * Z] = (int8_t)(uint8_t)(v[c] ^ 0x80u);
+  for (int c = 0; c < MC; ++c) g[c * Z] = (int8_t)(uint8_t)v[c];
 }
 
 // decode code c's segment of the grouped queue until it is drained; every thread of the
@@ -248,9 +249,9 @@ __device__ __forceinline__ void mc_run(const MultiArgs& a, int c, uint8_t* smem,
   while (any) {
     const long long* bt = btab ? btab + (size_t)iter * CT::mrow : nullptr;
     int pin0[CT::w[0]] = {};
-    code_layers<CT, 0, 0>(vb, c2v, bt, beta, cap, pin0, act);   // ends at a barrier
+    code_layers<CT, 0, 0, true>(vb, c2v, bt, beta, cap, pin0, act);   // ends at a barrier (two edges per op)
     // LDPCdecode.cpp:172-195: the syndrome of the slot's codeword
-    if (act && (code_syndrome<CT, 0, 0>(vb) & 0x80u)) flags[par * CPB + sl] = 1;
+    if (act && (code_syndrome<CT, 0, 0, true>(vb) & 0x80u)) flags[par * CPB + sl] = 1;
     __syncthreads();
     const bool cwfail = act && flags[par * CPB + sl] != 0;
     if (has_slot && r == 0) flags[(par ^ 1) * CPB + sl] = 0;   // read again two iterations on
