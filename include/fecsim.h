@@ -415,6 +415,13 @@ int fecsim_msim_stats(fecsim_msim *s, fecsim_sim_stats *per_code, unsigned long 
  * the per-entry (sigma, factor) pairs in code order, for inspection */
 int fecsim_msim_queue(const fecsim_msim *s, const int32_t **code, const unsigned long long **off, void **llr,
                       const float **sigma_factor);
+/* bytes of the packed LLR / posterior buffer that fecsim_msim_queue returns */
+int fecsim_msim_llr_bytes(const fecsim_msim *s, unsigned long long *bytes);
+/* Writes the channel LLRs of the next loop (the codewords and noise the next
+ * fecsim_msim_run decodes, which it generates again identically) into that buffer,
+ * without decoding or counting: a queue of fresh channel outputs for timing or tuning
+ * the queue decoder on (copy it aside first: the loop decodes in place). */
+int fecsim_msim_fill(fecsim_msim *s, void *stream);
 int fecsim_msim_reset(fecsim_msim *s, void *stream);
 void fecsim_msim_free(fecsim_msim *s);
 

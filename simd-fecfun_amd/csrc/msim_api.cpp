@@ -27,6 +27,7 @@ struct fecsim_msim {
   float* d_sf = nullptr;                    // [num_cw] (sigma, factor), in code order
   std::vector<uint32_t*> d_bits;            // per code [count][mcol][zw]
   void* d_llr = nullptr;                    // packed LLRs / posteriors
+  unsigned long long llr_bytes = 0;         // its size
   unsigned long long* d_ctr = nullptr;      // [ncode][4] cw, uncoded, bit err, cw err; [ncode*4] iterations; cw0
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -57,7 +58,8 @@ void free_msim(fecsim_msim* s) {
   (void)hipFree(s->d_ctr);
 }
 
-int enqueue(fecsim_msim* s, hipStream_t st) {
+// encoder and channel of every code for the loop at the device's codeword counter
+int enqueue_channel(fecsim_msim* s, hipStream_t st) {
   unsigned long long* cw0_dev = s->d_ctr + (size_t)s->ncode * kMStat + 1;
   hipError_t e;
   for (int c = 0; c < s->ncode; ++c) {
@@ -75,9 +77,17 @@ int enqueue(fecsim_msim* s, hipStream_t st) {
     la.sf = s->d_sf + 2 * (size_t)s->first[c];
     if ((e = launch_llr(s->elem, la, st)) != hipSuccess) return hip_fail(e, "llr launch");
   }
-  int rc = fecsim_multi_decode(s->m, s->p.num_cw, s->d_code, s->d_off, s->d_llr, s->d_llr, nullptr,
+  return FECSIM_OK;
+}
+
+int enqueue(fecsim_msim* s, hipStream_t st) {
+  hipError_t e;
+  int rc = enqueue_channel(s, st);
+  if (rc) return rc;
+  rc = fecsim_multi_decode(s->m, s->p.num_cw, s->d_code, s->d_off, s->d_llr, s->d_llr, nullptr,
                                s->d_ctr + (size_t)s->ncode * kMStat, st);
   if (rc) return rc;
+  unsigned long long* cw0_dev = s->d_ctr + (size_t)s->ncode * kMStat + 1;
   for (int c = 0; c < s->ncode; ++c) {
     if (!s->count[c]) continue;
     fecsim_henc* h = s->hencs[c];
@@ -153,7 +163,8 @@ int fecsim_msim_create(fecsim_henc* const* hencs, fecsim_multi* m, const fecsim_
   hipError_t e = hipMalloc((void**)&s->d_code, sizeof(int32_t) * p->num_cw);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_off, sizeof(unsigned long long) * p->num_cw);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_sf, sizeof(float) * sf.size());
-  if (e == hipSuccess) e = hipMalloc(&s->d_llr, (size_t)base * s->elem_bytes);
+  s->llr_bytes = base * (unsigned long long)s->elem_bytes;
+  if (e == hipSuccess) e = hipMalloc(&s->d_llr, (size_t)s->llr_bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_ctr, sizeof(unsigned long long) * ((size_t)n * kMStat + 2));
   for (int c = 0; c < n && e == hipSuccess; ++c) {
     uint32_t* b = nullptr;
@@ -236,6 +247,17 @@ int fecsim_msim_queue(const fecsim_msim* s, const int32_t** code, const unsigned
   if (llr) *llr = s->d_llr;
   if (sigma_factor) *sigma_factor = s->d_sf;
   return FECSIM_OK;
+}
+
+int fecsim_msim_llr_bytes(const fecsim_msim* s, unsigned long long* bytes) {
+  if (!s || !bytes) return fail(FECSIM_E_INVAL, "null argument");
+  *bytes = s->llr_bytes;
+  return FECSIM_OK;
+}
+
+int fecsim_msim_fill(fecsim_msim* s, void* stream) {
+  if (!s) return fail(FECSIM_E_INVAL, "null argument");
+  return enqueue_channel(s, (hipStream_t)stream);
 }
 
 int fecsim_msim_reset(fecsim_msim* s, void* stream) {

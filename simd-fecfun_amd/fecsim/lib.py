@@ -70,6 +70,8 @@ SIGNATURES = [
     ("fecsim_msim_run", _int, [_vp, _u32, _vp]),
     ("fecsim_msim_stats", _int, [_vp, _vp, C.POINTER(C.c_ulonglong)]),
     ("fecsim_msim_queue", _int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    ("fecsim_msim_llr_bytes", _int, [_vp, C.POINTER(C.c_ulonglong)]),
+    ("fecsim_msim_fill", _int, [_vp, _vp]),
     ("fecsim_msim_reset", _int, [_vp, _vp]),
     ("fecsim_msim_free", None, [_vp]),
     ("fecsim_gen_llr_var", _int, [_vp, _vp, C.c_float, _u64, _u64, _u32, _int, _vp, _int, _int, _vp, _vp, _vp]),

@@ -90,6 +90,20 @@ class MixedSim:
     def reset(self, stream=None):
         _L.check(_L.lib().fecsim_msim_reset(self._s, _stream(stream)), "fecsim_msim_reset")
 
+    def fill(self, stream=None):
+        """The next loop's channel LLRs into the queue's buffer, without decoding
+        (fecsim_msim_fill); llr() then views them."""
+        _L.check(_L.lib().fecsim_msim_fill(self._s, _stream(stream)), "fecsim_msim_fill")
+
+    def llr(self):
+        """torch int8 view of the packed LLR / posterior buffer (owned by the library)"""
+        import torch
+        llr = C.c_void_p()
+        nb = C.c_ulonglong()
+        _L.check(_L.lib().fecsim_msim_queue(self._s, None, None, C.byref(llr), None), "fecsim_msim_queue")
+        _L.check(_L.lib().fecsim_msim_llr_bytes(self._s, C.byref(nb)), "fecsim_msim_llr_bytes")
+        return torch.as_tensor(_DeviceArray(llr.value, nb.value, "|i1"), device="cuda")
+
     def stats(self):
         """(per-code list of dicts cw/uncoded_bit_err/bit_err/cw_err, summed iterations)"""
         from .sim import SimStats

@@ -181,3 +181,29 @@ def test_gpu_multi_code_graph_replays_and_static(cuda):
             assert np.array_equal(p[offs[i]:offs[i] + ln], want_post[offs[i]:offs[i] + ln]), (k, i)
         assert np.array_equal(it.cpu().numpy(), want_iters[:k]), k
         assert int(tot.item()) == int(want_iters[:k].sum())
+
+
+def test_gpu_mixed_sim_fill_is_the_next_loops_input(cuda):
+    """fecsim_msim_fill writes exactly the LLRs the next loop decodes: the queue decoder on
+    a copy of them gives the loop's in-place posteriors and its iteration total."""
+    import torch
+    from fecsim.multi import MixedSim
+    css, decs = _decoders()
+    hencs = [Henc.from_tables(cs.enc) for cs in css]
+    n = 3000
+    m = MultiDecoder(decs, n)
+    sim = MixedSim(hencs, m, n, ebn0=(1.0, 4.0), seed=31, use_graph=False)
+    sim.run(1)   # the fill is for the loop at the current codeword counter, not the first one
+    sim.reset()
+    sim.fill()
+    fresh = sim.llr().clone()
+    code, off, _, _ = sim.queue()
+    post = torch.zeros_like(fresh)
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    m.decode(code, off, fresh, post, iters=it)
+    sim.run(1)
+    torch.cuda.synchronize()
+    assert torch.equal(sim.llr(), post)
+    st, iters = sim.stats()
+    assert sum(s["cw"] for s in st) == n
+    assert iters == int(it.sum().item())
