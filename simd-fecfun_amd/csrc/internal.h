@@ -137,7 +137,7 @@ struct CountArgs {
   unsigned long long* counts;      // device [2]: bit errors, codeword errors (added to)
   uint32_t* cw_bit_err;            // device [num_cw] or null
   unsigned long long* cw_counter;  // device counter of codewords (added to), or null
-  FastDiv div_dz, div_z;           // by ndata*z and by z (set by the launcher)
+  FastDiv div_dz, div_z;           // by ndata*zw and by zw, zw = ceil(z / 32) (set by the launcher)
 };
 
 // Differentiable min-sum forward / adjoint (train.hip); F = float or double

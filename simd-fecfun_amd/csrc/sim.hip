@@ -262,7 +262,9 @@ __global__ void __launch_bounds__(256) k_llr(LlrArgs a) {
 
 // ---------------------------------------------------------------------------
 // error counting: a block takes a tile of `cpb` codewords and walks their data
-// bits flat; errors are rare, so they are tallied with LDS atomics
+// columns one 32-row word per thread: the word's hard decisions (value < 0,
+// LDPC_count_errors.cpp:84) against the data bits, one XOR and a popcount for 32
+// bits; errors are rare, so they are tallied per codeword with LDS atomics
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
@@ -270,38 +272,50 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   __shared__ unsigned long long s_mask[kCountMaxTile];
   __shared__ unsigned long long red[4];
   const T* post = reinterpret_cast<const T*>(a.post);
-  const uint32_t dz = (uint32_t)a.ndata * (uint32_t)a.z;
-  const uint32_t items = (uint32_t)a.cpb * dz;
+  const uint32_t zw = ((uint32_t)a.z + 31u) >> 5;            // words holding rows of a column
+  const uint32_t dw = (uint32_t)a.ndata * zw;                // words per codeword
+  const uint32_t items = (uint32_t)a.cpb * dw;
   for (int i = threadIdx.x; i < a.cpb; i += blockDim.x) { s_cnt[i] = 0; s_mask[i] = 0; }
   __syncthreads();
   unsigned long long bit_err = 0, cw_err = 0;
   const uint64_t ntiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
-  // item f of a tile = (codeword lc, data column c, row r), advanced by blockDim per step
+  // item f of a tile = (codeword lc, data column c, word w), advanced by blockDim per step
   const uint32_t dlc = fdiv(blockDim.x, a.div_dz);
-  const uint32_t dc = fdiv(blockDim.x - dlc * dz, a.div_z);
-  const uint32_t dr = blockDim.x - dlc * dz - dc * (uint32_t)a.z;
+  const uint32_t dc = fdiv(blockDim.x - dlc * dw, a.div_z);
+  const uint32_t dwd = blockDim.x - dlc * dw - dc * zw;
   const uint32_t lc0 = fdiv(threadIdx.x, a.div_dz);
-  const uint32_t c0 = fdiv(threadIdx.x - lc0 * dz, a.div_z);
-  const uint32_t r0 = threadIdx.x - lc0 * dz - c0 * (uint32_t)a.z;
+  const uint32_t c0 = fdiv(threadIdx.x - lc0 * dw, a.div_z);
+  const uint32_t w0 = threadIdx.x - lc0 * dw - c0 * zw;
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t cw_first = tile * a.cpb;
-    uint32_t lc = lc0, c = c0, r = r0;
+    uint32_t lc = lc0, c = c0, w = w0;
     for (uint32_t f = threadIdx.x; f < items; f += blockDim.x) {
       const uint64_t cw = cw_first + lc;
       if (cw < a.num_cw) {
-        const T v = post[(long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base + r];
-        const uint32_t hard = v < T(0) ? 1u : 0u;
-        const uint32_t bit = (a.bits[(cw * a.mcol + c) * a.col_words + (r >> 5)] >> (r & 31)) & 1u;
-        if (hard != bit) {
-          atomicAdd(&s_cnt[lc], 1u);
-          atomicOr(&s_mask[lc], 1ull << (r % a.num_par));
+        const T* col = post + ((long long)cw * a.cw_stride + (long long)c * a.col_stride + a.base + 32 * (long long)w);
+        const int nb = a.z - 32 * (int)w;   // rows of this word (32 but in a column's last word)
+        uint32_t hard = 0;
+#pragma unroll
+        for (int b = 0; b < 32; ++b)
+          if (b < nb) hard |= (col[b] < T(0) ? 1u : 0u) << b;
+        const uint32_t err =
+            (hard ^ a.bits[(cw * a.mcol + c) * a.col_words + w]) & zmask_word(a.z, (int)w);
+        if (err) {
+          atomicAdd(&s_cnt[lc], (uint32_t)__popc(err));
+          unsigned long long m = 1ull;   // rows r with an error mark bit r % num_par
+          if (a.num_par > 1) {
+            m = 0ull;
+            for (uint32_t x = err; x; x &= x - 1u)
+              m |= 1ull << ((32u * w + (uint32_t)__ffs(x) - 1u) % (uint32_t)a.num_par);
+          }
+          atomicOr(&s_mask[lc], m);
         }
       }
-      r += dr;
+      w += dwd;
       c += dc;
       lc += dlc;
-      if (r >= (uint32_t)a.z) {
-        r -= a.z;
+      if (w >= zw) {
+        w -= zw;
         ++c;
       }
       if (c >= (uint32_t)a.ndata) {
@@ -458,12 +472,13 @@ hipError_t launch_llr(int elem, const LlrArgs& a0, hipStream_t st) {
 hipError_t launch_count(int elem, const CountArgs& a0, hipStream_t st) {
   if (a0.num_cw == 0) return hipSuccess;
   CountArgs a = a0;
-  const uint32_t dz = (uint32_t)a.ndata * a.z;
-  a.cpb = (int)(2048 / dz);
+  const uint32_t zw = ((uint32_t)a.z + 31u) >> 5;
+  const uint32_t dw = (uint32_t)a.ndata * zw;   // 32-row words per codeword
+  a.cpb = (int)(1024 / dw);
   if (a.cpb < 1) a.cpb = 1;
   if (a.cpb > kCountMaxTile) a.cpb = kCountMaxTile;
-  a.div_dz = make_fastdiv(dz);
-  a.div_z = make_fastdiv((uint32_t)a.z);
+  a.div_dz = make_fastdiv(dw);
+  a.div_z = make_fastdiv(zw);
   const uint64_t tiles = ((uint64_t)a.num_cw + a.cpb - 1) / a.cpb;
   // at most 4 rounds of resident blocks; but at least 8 tiles a block beyond one round (each
   // block's two counter atomics at its end queue when the blocks of a short launch end
