@@ -176,9 +176,9 @@ int fecsim_sp_math(int fn, const float *x, float *y, uint32_t n, void *stream);
  * c, with results identical to fecsim_decode on that decoder.  All decoders need the
  * same element kind and mode, and their messages must fit one wave's LDS slice.
  * Two kernels serve a queue: the generic one (C2V messages in LDS, any codes) and,
- * when every decoder's code is compiled into it (802.11n 1/2 z27, 3/4 z54, 5/6 z81,
- * 802.11ad 13/16, 802.16e 2/3A at Z = 96; int8 min-sum, 1 <= beta, 0 <= cap, offset
- * tables with entries 1..127), one with the codes compiled in (C2V messages in
+ * when every decoder's code is compiled into it (every 802.11n code, rates 1/2 to 5/6 at
+ * Z = 27, 54 and 81, every 802.11ad code, and 802.16e 2/3A at Z = 96; int8 min-sum,
+ * 1 <= beta, 0 <= cap, offset tables with entries 1..127), one with the codes compiled in (C2V messages in
  * registers); the latter is the default then, and both give identical results. */
 typedef struct fecsim_multi fecsim_multi;
 

@@ -20,7 +20,11 @@ LDS_LIMIT = 160 * 1024
 # 256-thread workgroup, one row of each per thread; (code, lifting size if not the
 # standard's own)
 MULTI_SETS = [("11n 1/2 z27", None), ("11n 3/4 z54", None), ("11n 5/6 z81", None), ("11ad 13/16", None),
-              ("16e 2/3A", 96)]
+              ("16e 2/3A", 96),
+              # the rest of the 802.11n and 802.11ad families
+              ("11n 1/2 z54", None), ("11n 1/2 z81", None), ("11n 2/3 z27", None), ("11n 2/3 z54", None),
+              ("11n 2/3 z81", None), ("11n 3/4 z27", None), ("11n 3/4 z81", None), ("11n 5/6 z27", None),
+              ("11n 5/6 z54", None), ("11ad 1/2", None), ("11ad 5/8", None), ("11ad 3/4", None)]
 MULTI_THREADS = 256
 
 

@@ -25,9 +25,14 @@ def _say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _decoders(tables=False):
+# other codes compiled into k_multi_code (the rest of the 802.11n and 802.11ad families)
+CODES_B = [("11n 2/3 z27", None, 10), ("11n 5/6 z54", None, 12), ("11ad 1/2", None, 10), ("11ad 3/4", None, 8),
+           ("11n 1/2 z81", None, 6), ("11n 3/4 z27", None, 9), ("11ad 5/8", None, 10)]
+
+
+def _decoders(tables=False, codes_=CODES):
     css, decs = [], []
-    for name, zo, mi in CODES:
+    for name, zo, mi in codes_:
         cs = codes.gen_structs(name, simd_size=512, elem_size=8, z_over=zo)
         d = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", mi, 31.0, 1.0)
         if tables:   # per-(iteration, layer) offsets 1..3
@@ -43,7 +48,7 @@ def _queue(css, n, seed, weights=None, ebn0=(1.0, 4.0)):
     import torch
     rng = np.random.default_rng(seed)
     p = None if weights is None else np.asarray(weights, float) / np.sum(weights)
-    code = rng.choice(len(CODES), n, p=p).astype(np.int32)
+    code = rng.choice(len(css), n, p=p).astype(np.int32)
     parts, offs, base = [], np.zeros(n, np.int64), 0
     for c, cs in enumerate(css):
         idx = np.nonzero(code == c)[0]
@@ -100,7 +105,7 @@ def test_gpu_multi_code_kernel_is_chosen_for_compiled_codes(cuda):
     with pytest.raises(L.FecsimError):
         m.select_kernel(L.KERNEL_PAIR)
     # a code that is not compiled in: the generic kernel only
-    cs = codes.gen_structs("11n 1/2 z54", simd_size=512, elem_size=8)
+    cs = codes.gen_structs("ITU G.hn 1/2 z14", simd_size=512, elem_size=8)
     other = Decoder(Hdec.from_tables(cs.dec), "Vec64c_min", 10, 31.0, 1.0)
     m2 = MultiDecoder(decs[:2] + [other], 16)
     assert m2.kernel == L.KERNEL_GENERIC
@@ -130,6 +135,22 @@ def test_gpu_multi_code_matches_generic_and_single(n, seed, weights, cuda):
     assert tot == tot_g == int(iters.sum())
     if n <= 2000:
         _single(decs, parts, code, offs, post, iters)
+
+
+def test_gpu_multi_code_other_compiled_codes(cuda):
+    """a queue of seven more compiled codes (11n 2/3, 5/6 z54, 1/2 z81, 3/4 z27; 11ad 1/2,
+    3/4, 5/8): the compiled-code kernel equals the generic one and the single decoders"""
+    css, decs = _decoders(codes_=CODES_B)
+    n = 4000
+    code, offs, packed, parts = _queue(css, n, 41, ebn0=(0.5, 4.5))
+    _say("multi code kernel, other codes: per code", np.bincount(code, minlength=len(CODES_B)).tolist())
+    m = MultiDecoder(decs, n)
+    assert m.kernel == L.KERNEL_CODE
+    post, iters, tot = _run(m, code, offs, packed)
+    m.select_kernel(L.KERNEL_GENERIC)
+    post_g, iters_g, tot_g = _run(m, code, offs, packed)
+    assert np.array_equal(post, post_g) and np.array_equal(iters, iters_g) and tot == tot_g
+    _single(decs, parts, code, offs, post, iters)
 
 
 def test_gpu_multi_code_offset_tables(cuda):
