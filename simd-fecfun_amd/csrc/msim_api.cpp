@@ -30,6 +30,11 @@ struct fecsim_msim {
   unsigned long long llr_bytes = 0;         // its size
   unsigned long long* d_ctr = nullptr;      // [ncode][4] cw, uncoded, bit err, cw err; [ncode*4] iterations; cw0
   hipStream_t cap_stream = nullptr;
+  // per-code launches (encoder and channel, counter) fork over side streams (default
+  // priority), code c on side[c % side.size()], and join back before the decoder / the advance
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -37,6 +42,7 @@ struct fecsim_msim {
 namespace {
 
 constexpr int kMStat = 4;
+constexpr int kMaxSideStreams = 8;
 
 uint64_t mix64(uint64_t z) {   // SplitMix64 output function (Steele, Lea, Flood 2014)
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -50,6 +56,9 @@ void free_msim(fecsim_msim* s) {
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
+  for (hipStream_t x : s->side) (void)hipStreamDestroy(x);
+  for (hipEvent_t x : s->joins) (void)hipEventDestroy(x);
+  if (s->fork) (void)hipEventDestroy(s->fork);
   (void)hipFree(s->d_code);
   (void)hipFree(s->d_off);
   (void)hipFree(s->d_sf);
@@ -58,12 +67,34 @@ void free_msim(fecsim_msim* s) {
   (void)hipFree(s->d_ctr);
 }
 
+// the side streams start after the work already on st
+int fork_sides(fecsim_msim* s, hipStream_t st) {
+  hipError_t e = hipEventRecord(s->fork, st);
+  for (size_t k = 0; k < s->side.size() && e == hipSuccess; ++k) e = hipStreamWaitEvent(s->side[k], s->fork, 0);
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "fork to the side streams");
+}
+
+// st continues after everything on the side streams
+int join_sides(fecsim_msim* s, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < s->side.size() && e == hipSuccess; ++k) {
+    e = hipEventRecord(s->joins[k], s->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, s->joins[k], 0);
+  }
+  return e == hipSuccess ? FECSIM_OK : hip_fail(e, "join from the side streams");
+}
+
+hipStream_t side_of(const fecsim_msim* s, int c) { return s->side[(size_t)c % s->side.size()]; }
+
 // encoder and channel of every code for the loop at the device's codeword counter
-int enqueue_channel(fecsim_msim* s, hipStream_t st) {
+int enqueue_channel(fecsim_msim* s, hipStream_t st0) {
   unsigned long long* cw0_dev = s->d_ctr + (size_t)s->ncode * kMStat + 1;
   hipError_t e;
+  int rc = fork_sides(s, st0);
+  if (rc) return rc;
   for (int c = 0; c < s->ncode; ++c) {
     if (!s->count[c]) continue;
+    const hipStream_t st = side_of(s, c);
     fecsim_henc* h = s->hencs[c];
     unsigned long long* ctr = s->d_ctr + (size_t)c * kMStat;
     // codeword indices of code c this loop: *cw0_dev + first[c] .. + count[c]
@@ -77,7 +108,7 @@ int enqueue_channel(fecsim_msim* s, hipStream_t st) {
     la.sf = s->d_sf + 2 * (size_t)s->first[c];
     if ((e = launch_llr(s->elem, la, st)) != hipSuccess) return hip_fail(e, "llr launch");
   }
-  return FECSIM_OK;
+  return join_sides(s, st0);
 }
 
 int enqueue(fecsim_msim* s, hipStream_t st) {
@@ -88,6 +119,7 @@ int enqueue(fecsim_msim* s, hipStream_t st) {
                                s->d_ctr + (size_t)s->ncode * kMStat, st);
   if (rc) return rc;
   unsigned long long* cw0_dev = s->d_ctr + (size_t)s->ncode * kMStat + 1;
+  if ((rc = fork_sides(s, st))) return rc;
   for (int c = 0; c < s->ncode; ++c) {
     if (!s->count[c]) continue;
     fecsim_henc* h = s->hencs[c];
@@ -95,8 +127,9 @@ int enqueue(fecsim_msim* s, hipStream_t st) {
     CountArgs ca;
     count_args(h, s->count[c], h->h.zw, s->d_bits[c], 0, static_cast<char*>(s->d_llr) + s->llr_base[c] * s->elem_bytes,
                ctr + 2, nullptr, ctr + 0, &ca);
-    if ((e = launch_count(s->elem, ca, st)) != hipSuccess) return hip_fail(e, "count launch");
+    if ((e = launch_count(s->elem, ca, side_of(s, c))) != hipSuccess) return hip_fail(e, "count launch");
   }
+  if ((rc = join_sides(s, st))) return rc;
   if ((e = launch_advance(cw0_dev, s->p.num_cw, st)) != hipSuccess) return hip_fail(e, "advance launch");
   return FECSIM_OK;
 }
@@ -175,6 +208,16 @@ int fecsim_msim_create(fecsim_henc* const* hencs, fecsim_multi* m, const fecsim_
   if (e == hipSuccess) e = hipMemcpy(s->d_off, off.data(), sizeof(unsigned long long) * p->num_cw, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(s->d_sf, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(s->d_ctr, 0, sizeof(unsigned long long) * ((size_t)n * kMStat + 2));
+  const int nside = std::min(n, kMaxSideStreams);
+  for (int k = 0; k < nside && e == hipSuccess; ++k) {
+    hipStream_t x = nullptr;
+    hipEvent_t j = nullptr;
+    e = hipStreamCreateWithFlags(&x, hipStreamNonBlocking);
+    if (e == hipSuccess) s->side.push_back(x);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if (e == hipSuccess) s->joins.push_back(j);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     free_msim(s);
