@@ -227,3 +227,30 @@ def test_code_tables_compiled_in():
         assert w == list(d.row_weights) and col == list(d.row_active_cols) and sh == list(d.row_offset)
         assert (3 * mcol + 1) * z <= gen_code_tables.LDS_LIMIT
     assert tb["802.3ca"][:2] == [256, 69] and len(tb["802.3ca"][3]) == 275
+
+
+def test_multi_code_tables_compiled_in():
+    """The codes compiled into the mixed-queue kernel are exactly the decoder tables
+    gen_structs writes, every 802.11n and 802.11ad code and 16e 2/3A at Z = 96, and each
+    fits the kernel's layout: at least one codeword slot per 256 threads, a slot's LDS
+    (two copies and a gap per column) within the instruction's 15-bit immediate offsets,
+    and the generated header's match list in the same order."""
+    import os
+    from fecsim import gen_code_tables
+    mt = gen_code_tables.multi_tables()
+    names = [t[0] for t in mt]
+    assert len(set(names)) == len(names) == 17
+    assert {"11n %s z%d" % (r, z) for r in ("1/2", "2/3", "3/4", "5/6") for z in (27, 54, 81)} <= set(names)
+    assert {"11ad 1/2", "11ad 5/8", "11ad 3/4", "11ad 13/16", "16e 2/3A Z=96"} <= set(names)
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "simd-fecfun_amd", "csrc", "code_tables.h")).read()
+    for i, (name, z, mcol, w, col, sh) in enumerate(mt):
+        ms, zo = (name[:-len(" Z=96")], 96) if name.endswith(" Z=96") else (name, None)
+        d = codes.gen_structs(ms, simd_size=512, elem_size=8, z_over=zo).dec
+        assert (z, mcol) == (d.z, d.mcol)
+        assert w == list(d.row_weights) and col == list(d.row_active_cols) and sh == list(d.row_offset)
+        assert 1 <= gen_code_tables.MULTI_THREADS // z
+        assert ((3 * mcol + 1) * z + 15) // 16 * 16 < 32768
+        assert min(w) >= 2
+        assert "struct MultiTab%d {  // %s" % (i, name) in hdr
+        assert "if (code_tab_match<MultiTab%d>(z, mcol, mrow, w, col, sh)) return %d;" % (i, i) in hdr
