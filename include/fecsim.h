@@ -397,7 +397,10 @@ void fecsim_sim_free(fecsim_sim *s);
  * ebn0_hi] dB (Es/N0 = Eb/N0 + 10 log10(K/N) of its code).  Every loop generates new
  * codewords and noise for all entries (encoder + channel of each code, LLRs of the
  * decoders' element type), decodes the interleaved queue with fecsim_multi_decode and
- * counts errors per code; with use_graph the loop is one replayed HIP graph. */
+ * counts errors per code; with use_graph the loop is one replayed HIP graph.  The
+ * per-code launches run on up to 8 side streams of the handle, forked from and joined
+ * back into the caller's stream.  No reference counterpart (each reference process
+ * simulates one code). */
 typedef struct fecsim_msim fecsim_msim;
 typedef struct {
     uint32_t num_cw;
